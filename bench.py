@@ -1,0 +1,141 @@
+"""Benchmark of the FVM time-stepping core on one MI355X.
+
+Workload (default): the 512x384x256-cell shoebox (dx = 0.01 m, 16-cell cPML on +x,
+five ADE walls drawn from three seeded material types, K = 4 real poles + L = 2
+complex pairs, Ricker 2 kHz source, 8 receivers), FP32 field storage. A step is one
+full SSP-RK3 step (3 flux stages + sources + immersed boundary) over the room.
+
+Prints ONE JSON line: Mcell-updates/s (room cells x steps / HIP-event time of the
+step loop), the flux-stage kernel's achieved algorithmic HBM bandwidth vs the 8 TB/s
+peak, the IR relative L2 of this run vs the FP64 path on the same room and window,
+and the FP64 CPU oracle timed on a bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-updates/s + % HBM roofline; batched-source & adjoint vs plain, same GPU"
+ESZ = {"fp64": 8, "fp32": 4, "fp16": 2}
+DTYPE = {"fp64": "f64", "fp32": "f32", "fp16": "f16-storage/f32-math"}
+
+
+def log(msg: str) -> None:
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--room", default="512,384,256")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline sample (0: skip)")
+    ap.add_argument("--no-fp64-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the FVM core has no multi-GPU path in this build "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        return 2
+
+    from fases_amd import workloads as W
+
+    room = tuple(int(v) for v in args.room.split(","))
+    total = args.warmup + args.steps
+    t0 = time.time()
+    wl = W.config(2, room=room, nsteps=total, precision=args.precision)
+    sc = wl.scene
+    log("scene %s padded %s built in %.1f s" % (room, sc.dims, time.time() - t0))
+
+    sv = W.make_solver(wl, args.precision)
+    info = sv.info()
+    log("device bytes %.2f GB, boundary cells %d, updated cells %d" % (info["dev_bytes"] / 1e9, info["n_bnd"],
+                                                                       info["n_upd"]))
+    ir_w, _, ms_w, _ = sv.run(args.warmup)
+    log("warmup %d steps %.1f ms" % (args.warmup, ms_w))
+    t1 = time.time()
+    ir_t, _, ms, kms = sv.run(args.steps, kernel_timing=True)
+    wall = time.time() - t1
+    sv.close()
+    cells = wl.room_cells
+    mcells = cells * args.steps / (ms * 1e-3) / 1e6
+    log("timed %d steps: %.1f ms (%.3f ms/step), flux kernels %.1f ms, host wall %.2f s -> %.0f Mcell/s"
+        % (args.steps, ms, ms / args.steps, kms, wall, mcells))
+
+    esz = ESZ[args.precision]
+    launches = 3 * args.steps
+    alg_bytes_launch = info["n_upd"] * 32 * esz / 3.0
+    avg_launch_s = kms * 1e-3 / launches
+    achieved = alg_bytes_launch / avg_launch_s / 1e9
+    traffic = None
+    pmc = ROOT / "profiles" / ("flux_stage_%s_%s_traffic.json" % ("x".join(map(str, room)), args.precision))
+    if pmc.exists():
+        try:
+            traffic = json.loads(pmc.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    ir_err = None
+    if not args.no_fp64_check and args.precision != "fp64":
+        sv64 = W.make_solver(wl, "fp64")
+        ir64 = np.concatenate([sv64.run(args.warmup)[0], sv64.run(args.steps)[0]], axis=1)
+        sv64.close()
+        ir = np.concatenate([ir_w, ir_t], axis=1)
+        ir_err = float(np.linalg.norm(ir - ir64) / max(np.linalg.norm(ir64), 1e-300))
+        log("IR rel L2 vs FP64 path over %d steps: %.3e (|IR|max %.3e)" % (total, ir_err, np.abs(ir64).max()))
+
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t2 = time.time()
+        O.run_workload(wl, nsteps=args.cpu_steps)
+        el = time.time() - t2
+        cpu = {"value": round(cells * args.cpu_steps / el / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
+               "kind": "port",
+               "sample": "FP64 oracle (OpenMP) on the same %s room, first %d steps, %.1f s"
+                         % ("x".join(map(str, room)), args.cpu_steps, el)}
+        log("cpu baseline %.2f Mcell/s on %d threads" % (cpu["value"], threads))
+
+    out = {
+        "metric": METRIC,
+        "value": round(mcells, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.precision],
+        "data": "synthetic",
+        "config": {"workload": "shoebox %s cells dx=0.01 m, %s storage, 16-cell cPML +x, 5 ADE walls "
+                               "(3 types, K=4 L=2), Ricker 2 kHz, 8 receivers" % ("x".join(map(str, room)),
+                                                                                   args.precision.upper()),
+                   "room_cells": cells, "padded_cells": info["n_cells"], "boundary_cells": info["n_bnd"],
+                   "updated_cells": info["n_upd"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "flux stage (WENO5-Z + Godunov + SSP-RK3 + cPML)",
+                     "alg_bytes_per_launch": int(alg_bytes_launch)},
+        "ir_rel_l2_vs_fp64": ir_err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,46 @@
+// Internal host-side declarations shared by the C++ translation units of libfases_core.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fases {
+
+// Thread-local message of the last failing call made without a context.
+void set_global_error(const std::string &msg);
+const char *global_error();
+
+// Single-precision 3-vector. The geometry pipeline is evaluated in float32 with the
+// operand order of 3D-solver/include/parsing3D.h so that S and oid come out bit-identical.
+struct F3 {
+    float x, y, z;
+};
+
+struct Triangle {
+    int a, b, c;   // vertex indices
+    int group;     // mesh group (material) index
+};
+
+struct Mesh {
+    std::vector<F3> verts;
+    std::vector<Triangle> tris;
+    std::vector<std::string> groups;   // group id == index
+    F3 lo{0, 0, 0}, hi{0, 0, 0};       // raw bounding box
+    F3 offset{0, 0, 0};                // placement shift (+3h padding)
+};
+
+struct Scene {
+    Mesh mesh;
+    float h = 0.f;
+    int Nx = 0, Ny = 0, Nz = 0;
+    float Lx = 0.f, Ly = 0.f, Lz = 0.f;   // padded extent
+    std::vector<float> S;
+    std::vector<uint8_t> oid;
+};
+
+bool mesh_from_obj(const std::string &path, Mesh &m, std::string &err);
+void mesh_shoebox(int nx, int ny, int nz, float h, Mesh &m);
+bool scene_build(Scene &sc, std::string &err);   // placement, grid dims, SDF, OID
+
+}  // namespace fases

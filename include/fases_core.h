@@ -1,0 +1,198 @@
+/*
+ * fases_core.h -- C ABI of the MI355X-native FASES 3D-solver time-stepping core.
+ *
+ * Plain C: pointers, sizes and ints only. Every function returns 0 on success
+ * or a negative FASES_E_* code, and never exits the process. Host arrays passed
+ * in belong to the caller: they are copied and never retained. Device memory
+ * belongs to the context. A context is single-threaded and owns one HIP stream
+ * on an explicit device.
+ *
+ * Reference interfaces replaced (vitorotri/fases, paths relative to the repo):
+ *   scene     <- 3D-solver/include/parsing3D.h (parse_obj3d :690, build_sdf :860,
+ *                build_oid_dense :1145) and common/filterABC.h (build_boundary_map :95)
+ *   formats   <- 3D-solver/core/main3D.cu (config.txt :114-128, poles.txt :592-603,
+ *                src.txt :861-906, rec.txt :919-953, waveforms :963-1124,
+ *                IR/VEL files :1242-1256,1397-1414, header.txt :1441-1449) and
+ *                common/filterABC.h read_fit_file :31-92
+ *   solver    <- 3D-solver/core/main3D.cu time loop :1280-1320 driving
+ *                3D-solver/core/functions3D.cuh kernel_WENO_stage{0,1,2} :1086-1327,
+ *                kernel_inject_src_SSP :1039-1047, kernel_apply_BC_immersed_sdf
+ *                :462-565 (ADE_solver :426-459, cPML :681-756)
+ */
+#ifndef FASES_CORE_H
+#define FASES_CORE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FASES_OK        0
+#define FASES_E_ARG    -1   /* bad argument / size                               */
+#define FASES_E_IO     -2   /* file cannot be opened / written                   */
+#define FASES_E_PARSE  -3   /* malformed input file                              */
+#define FASES_E_HIP    -4   /* HIP runtime error                                 */
+#define FASES_E_NOMEM  -5   /* host or device allocation failed                  */
+#define FASES_E_STATE  -6   /* call out of order (e.g. run before geometry)      */
+#define FASES_E_RANGE  -7   /* receiver / source outside the grid                */
+#define FASES_E_NONFINITE -8 /* receiver sample became NaN/Inf                   */
+
+/* storage precision of the p, vx, vy, vz fields */
+#define FASES_FP64 0        /* fields, flux math, ADE states: FP64               */
+#define FASES_FP32 1        /* fields + flux math FP32; ADE states, receivers FP64 */
+#define FASES_FP16 2        /* fields FP16 with per-field power-of-two scale, math FP32;
+                               ADE states, receivers FP64                          */
+
+/* source waveform selection (config line 16, main3D.cu:966) */
+#define FASES_SRC_PCS    0
+#define FASES_SRC_RICKER 1
+#define FASES_SRC_SINE   2
+
+typedef struct fases_ctx   fases_ctx;
+typedef struct fases_scene fases_scene;
+
+typedef struct fases_params {
+    int    device;          /* HIP device index                                        */
+    int    precision;       /* FASES_FP64 / FASES_FP32 / FASES_FP16                    */
+    double h;               /* cell size [m]                                           */
+    double dt;              /* time step [s]  (reference: CFL*h/c, main3D.cu:730)      */
+    double c;               /* speed of sound [m/s]                                    */
+    double rho;             /* density [kg/m^3]  (reference: 1.225, main3D.cu:454)     */
+    double tol;             /* immersed band thickness (reference: 3h, main3D.cu:224)  */
+    float  pml_keys[6];     /* U(z+) D(z-) R(x+) L(x-) F(y+) B(y-): config lines 7..12 */
+    int    pml_cells;       /* PML depth d (reference PML_CELLS = 20)                  */
+    double pml_sigma_max;   /* <= 0: reference 0.5 c (m+1) ln(1/R0)/(d h), R0=1e-7,m=2 */
+    double pml_kappa_max;   /* reference 5                                             */
+    double pml_alpha_max;   /* reference 20 pi rad/s                                   */
+    int    fp16_rescale_every; /* FP16: steps between per-field scale updates (100)    */
+    int    use_graph;       /* 1: replay the step loop as a captured hipGraph          */
+} fases_params;
+
+/* Fills *p with the reference defaults (FP64, device 0, PML depth 20, ...). */
+int fases_params_default(fases_params *p);
+
+/* ------------------------------------------------------------------------- */
+/* Formats (host C++)                                                        */
+/* ------------------------------------------------------------------------- */
+
+/* config.txt: first float of every line after '#'-comments are cut (main3D.cu:114-128).
+ * Writes up to maxn values into vals, the count into *n. */
+int fases_read_config(const char *path, float *vals, int maxn, int *n);
+
+/* poles.txt "K_m L_m", each floored to 1 (main3D.cu:592-603). */
+int fases_read_poles(const char *path, int *K, int *L);
+
+/* One .fit file into material slot m of arrays laid out [m*K + k] / [m*L + l]
+ * (filterABC.h read_fit_file:31-92). K, L are the poles.txt maxima. */
+int fases_read_fit(const char *path, int m, int K, int L,
+                   double *Rinf, double *A, double *zeta,
+                   double *B, double *C, double *alpha, double *beta);
+
+/* src.txt rows "x y z Lp,1m [f0]" -> cell = (int)(x/h)+3 per axis, sources outside
+ * the grid are dropped (main3D.cu:861-906). Arrays hold maxn entries. */
+int fases_read_sources(const char *path, float h, int Nx, int Ny, int Nz, float default_f0,
+                       int maxn, int64_t *cell, float *lp, float *f0, float *xyz, int *n);
+
+/* rec.txt rows "x y z" -> cell; a receiver outside the grid is FASES_E_RANGE
+ * (main3D.cu:919-953). */
+int fases_read_receivers(const char *path, float h, int Nx, int Ny, int Nz,
+                         int maxn, int64_t *cell, float *xyz, int *n);
+
+/* Shared source waveform u_s[0..2*Nt] sampled at dt/2 (main3D.cu:963-1058):
+ * FASES_SRC_PCS (f = config line 17, f_cutoff for the pulse width) or
+ * FASES_SRC_RICKER (f = peak frequency). dt is the float time step. */
+int fases_source_waveform(int src_opt, float dt, double f, float f_cutoff, int64_t Nt, double *u_s);
+
+/* Analytic 1 m pressure per unit drive p_ref[0..2*Nt] and its peak (main3D.cu:1079-1094). */
+int fases_source_pref(const double *u_s, int64_t Nt, float dt, float h, float rho,
+                      double *p_ref, double *p_peak);
+
+/* Little-endian float32 receiver file (IR<r>_<s>.bin / VEL, main3D.cu:1397-1414):
+ * n values of x rounded to float32. */
+int fases_write_f32(const char *path, const double *x, int64_t n);
+
+/* ------------------------------------------------------------------------- */
+/* Scene: OBJ -> padded grid, signed distance S, material id oid (host C++)  */
+/* ------------------------------------------------------------------------- */
+
+/* OBJ file -> mesh shifted by +3h, narrow-band SDF and dense OID
+ * (parsing3D.h parse_obj3d:690, build_sdf:860 CPU path, build_oid_dense_cpu:955). */
+int fases_scene_load_obj(const char *path, float h, fases_scene **out);
+
+/* Axis-aligned shoebox of nx*ny*nz room cells of size h: the same mesh an OBJ with
+ * walls west/east/south/north/floor/ceiling (x-,x+,y-,y+,z-,z+) would give. */
+int fases_scene_shoebox(int nx, int ny, int nz, float h, fases_scene **out);
+
+/* Padded grid dims, number of mesh groups (materials), padded extent [m]. */
+int fases_scene_info(const fases_scene *s, int *Nx, int *Ny, int *Nz, int *n_groups, float *L3);
+
+/* Name of mesh group g (0-based; material m = g, oid = g+1). */
+int fases_scene_group_name(const fases_scene *s, int g, char *buf, int buflen);
+
+/* Copies S [Nx*Ny*Nz float] and oid [Nx*Ny*Nz uint8] (either may be NULL). */
+int fases_scene_get(const fases_scene *s, float *S, uint8_t *oid);
+
+void fases_scene_free(fases_scene *s);
+
+/* Compact ADE index: bmap[c] = running count where oid not in {0, 0xFF}, else -1
+ * (filterABC.h build_boundary_map:95-108). */
+int fases_boundary_map(const uint8_t *oid, int64_t N, int32_t *bmap, int64_t *nB);
+
+/* ------------------------------------------------------------------------- */
+/* Solver context (HIP)                                                      */
+/* ------------------------------------------------------------------------- */
+
+int fases_create(const fases_params *p, fases_ctx **out);
+int fases_destroy(fases_ctx *ctx);
+
+/* Message of the last failing call on ctx (ctx may be NULL: last fases_create / scene /
+ * format error of the calling thread). */
+const char *fases_last_error(const fases_ctx *ctx);
+
+/* Padded-grid geometry: S (signed distance, >0 in air) and oid (1-based material id,
+ * 0xFF outside the band). Builds the update mask (interior, S >= h), the boundary-cell
+ * list (bmap order) and the PML tables; uploads everything. */
+int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int Nx, int Ny, int Nz);
+
+/* ADE materials, arrays [m*K + k] and [m*L + l] (main3D.cu:575-668). Call after
+ * fases_set_geometry; allocates the FP64 pole states. */
+int fases_set_materials(fases_ctx *ctx, int nMat, int K, int L,
+                        const double *Rinf, const double *A, const double *zeta,
+                        const double *B, const double *C, const double *alpha, const double *beta);
+
+/* Point sources (kernel_inject_src_SSP, functions3D.cuh:1039): cell indices, scaled
+ * amplitude sval, option PCS/Ricker (shared u_s[n_samp], n_samp >= 2*nsteps+1 of every
+ * later run) or SINE (per-source freq [Hz]). */
+int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *sval,
+                      int src_opt, const double *u_s, int64_t n_samp, const double *freq);
+
+/* Receiver cells; each run records p (and vx,vy,vz) after every step. */
+int fases_set_receivers(fases_ctx *ctx, int n, const int64_t *cell);
+
+/* Sets the state U^n on every cell of all three field sets (NULL field = zero) and
+ * clears ADE/PML states and the step counter. */
+int fases_set_fields(fases_ctx *ctx, const double *p, const double *u, const double *v, const double *w);
+
+/* Copies the current state U^n (padded grid, FP64 on the host). Any pointer may be NULL. */
+int fases_get_fields(const fases_ctx *ctx, double *p, double *u, double *v, double *w);
+
+/* Zero fields, ADE and PML states, step counter 0. */
+int fases_reset(fases_ctx *ctx);
+
+/* Advances nsteps SSP-RK3 steps from the current step counter. ir [n_rec*nsteps]
+ * (row r = receiver r) receives p after each step, vel [n_rec*nsteps*3] (nullable)
+ * vx,vy,vz. *elapsed_ms (nullable) = HIP-event time of the step loop alone.
+ * *kernel_ms (nullable) = summed HIP-event time of the flux-stage launches (events are
+ * recorded around each of them only when kernel_ms is non-NULL). */
+int fases_run(fases_ctx *ctx, int64_t nsteps, double *ir, double *vel,
+              double *elapsed_ms, double *kernel_ms);
+
+/* Sizes: padded cells, boundary cells, room (updated) cells, device bytes in use. */
+int fases_info(const fases_ctx *ctx, int64_t *n_cells, int64_t *n_bnd, int64_t *n_upd,
+               int64_t *dev_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASES_CORE_H */

@@ -1,0 +1,161 @@
+"""HIP path vs the FP64 oracle on the same seeded inputs (GPU required).
+
+Bars (stated per test): FP64 fields relative L2 of receiver IRs <= 1e-10;
+FP32 storage/arithmetic <= 1e-5. Boundary-cell classification is bit-exact upstream
+(tests/test_geometry.py), so both sides see identical S / oid.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from fases_amd import _lib
+from fases_amd import workloads as W
+from fases_amd.solver import Materials, SolverParams
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_TOL = 1e-5
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def announce(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def small_ade(seed=7, room=(40, 32, 24), nsteps=240, n_rec=3, keys=None, pml_cells=8, K=4, L=2, **kw):
+    wl = W.ade_room(room=room, h=0.02, nsteps=nsteps, seed=seed, n_rec=n_rec, K=K, L=L, pml_cells=pml_cells, **kw)
+    if keys is not None:
+        wl = dataclasses.replace(wl, params=dataclasses.replace(wl.params, pml_keys=keys))
+    return wl
+
+
+def gpu_run(wl, precision, nsteps=None, want_vel=True):
+    sv = W.make_solver(wl, precision)
+    ir, vel, _, _ = sv.run(wl.nsteps if nsteps is None else nsteps, want_vel=want_vel)
+    fields = sv.get_fields()
+    sv.close()
+    return ir, vel, fields
+
+
+def test_fp64_rigid_box_initial_pulse_full():
+    """64x48x40 rigid box, Gaussian pulse, 600 steps (the CPU-runnable benchmark room)."""
+    wl = W.config(0)
+    announce("fp64 rigid box %s %d steps" % (wl.dims, wl.nsteps))
+    ir, vel, fields = gpu_run(wl, "fp64")
+    ref = O.run_workload(wl, want_vel=True, want_fields=True)
+    assert rel_l2(ir, ref["ir"]) <= FP64_TOL
+    assert rel_l2(vel, ref["vel"]) <= FP64_TOL
+    assert rel_l2(fields[0], ref["fields"][0]) <= FP64_TOL
+    assert np.abs(ir).max() > 1e-3
+
+
+def test_fp64_ade_pml_ricker_small():
+    wl = small_ade()
+    announce("fp64 ADE+PML(+x) %s %d steps" % (wl.dims, wl.nsteps))
+    ir, vel, fields = gpu_run(wl, "fp64")
+    ref = O.run_workload(wl, want_vel=True, want_fields=True)
+    assert rel_l2(ir, ref["ir"]) <= FP64_TOL
+    assert rel_l2(vel, ref["vel"]) <= FP64_TOL
+    for f in range(4):
+        assert rel_l2(fields[f], ref["fields"][f]) <= FP64_TOL
+
+
+def test_fp64_pml_all_faces_and_complex_only_materials():
+    wl = small_ade(seed=3, room=(30, 28, 26), nsteps=200, keys=(1, 1, 1, 1, 1, 1), pml_cells=6, K=1, L=3)
+    announce("fp64 PML all faces %s" % (wl.dims,))
+    ir, vel, fields = gpu_run(wl, "fp64")
+    ref = O.run_workload(wl, want_vel=True, want_fields=True)
+    assert rel_l2(ir, ref["ir"]) <= FP64_TOL
+    for f in range(4):
+        assert rel_l2(fields[f], ref["fields"][f]) <= FP64_TOL
+
+
+def test_fp32_ade_pml_ricker_small():
+    wl = small_ade(nsteps=300)
+    announce("fp32 ADE+PML %s %d steps" % (wl.dims, wl.nsteps))
+    ir, _, _ = gpu_run(wl, "fp32", want_vel=False)
+    ref = O.run_workload(wl)
+    err = rel_l2(ir, ref["ir"])
+    print("fp32 IR rel L2 = %.3e" % err)
+    assert err <= FP32_TOL
+
+
+def test_fp32_rigid_box():
+    wl = W.config(0)
+    announce("fp32 rigid box 600 steps")
+    ir, _, _ = gpu_run(wl, "fp32", want_vel=False)
+    ref = O.run_workload(wl)
+    err = rel_l2(ir, ref["ir"])
+    print("fp32 rigid IR rel L2 = %.3e" % err)
+    assert err <= FP32_TOL
+
+
+@pytest.mark.parametrize("opt", [_lib.FASES_SRC_PCS, _lib.FASES_SRC_SINE])
+def test_fp64_source_options(opt):
+    wl = small_ade(seed=5, room=(28, 24, 20), nsteps=150, n_rec=2)
+    if opt == _lib.FASES_SRC_PCS:
+        u_s = W.source_waveform(opt, wl.params.dt, 25.0, 1400.0, wl.nsteps)
+        wl = dataclasses.replace(wl, src_opt=opt, u_s=u_s, src_sval=[3.0e4])
+        freq = None
+    else:
+        wl = dataclasses.replace(wl, src_opt=opt, u_s=None, src_sval=[2.0])
+        freq = [900.0]
+    announce("fp64 source option %d" % opt)
+    sv = W.make_solver(wl, "fp64")
+    sv.set_sources([wl.cell(s) for s in wl.src_ijk], wl.src_sval, opt, wl.u_s, freq)
+    ir, _, _, _ = sv.run(wl.nsteps)
+    sc = wl.scene
+    ref = O.run(sc.S, sc.oid, sc.dims, wl.params, wl.materials, wl.nsteps, src_cells=[wl.cell(s) for s in wl.src_ijk],
+                sval=wl.src_sval, src_opt=opt, u_s=wl.u_s, freq=freq, rec_cells=[wl.cell(r) for r in wl.rec_ijk])
+    assert np.abs(ref["ir"]).max() > 0
+    assert rel_l2(ir, ref["ir"]) <= FP64_TOL
+
+
+def test_run_chaining_is_exact():
+    """run(a) + run(b) continues the same trajectory as run(a+b), bit for bit."""
+    wl = small_ade(seed=9, room=(24, 20, 18), nsteps=120, n_rec=2)
+    announce("chaining")
+    sv = W.make_solver(wl, "fp64")
+    a, _, _, _ = sv.run(50)
+    b, _, _, _ = sv.run(70)
+    sv.set_fields(wl.initial_pressure())
+    full, _, _, _ = sv.run(120)
+    assert np.array_equal(np.concatenate([a, b], axis=1), full)
+
+
+def test_same_cell_sources_merge():
+    """Two sources in one cell act as one source with the summed drive."""
+    wl = small_ade(seed=2, room=(24, 20, 18), nsteps=80, n_rec=2)
+    announce("same-cell sources")
+    sv = W.make_solver(wl, "fp64")
+    c = wl.cell(wl.src_ijk[0])
+    sv.set_sources([c, c], [0.4 * wl.src_sval[0], 0.6 * wl.src_sval[0]], wl.src_opt, wl.u_s)
+    ir2, _, _, _ = sv.run(wl.nsteps)
+    sc = wl.scene
+    ref = O.run(sc.S, sc.oid, sc.dims, wl.params, wl.materials, wl.nsteps, src_cells=[c], sval=wl.src_sval,
+                src_opt=wl.src_opt, u_s=wl.u_s, rec_cells=[wl.cell(r) for r in wl.rec_ijk])
+    assert rel_l2(ir2, ref["ir"]) <= FP64_TOL
+
+
+def test_error_paths_on_device():
+    wl = small_ade(seed=1, room=(16, 16, 16), nsteps=10, n_rec=1)
+    announce("error paths")
+    sv = W.make_solver(wl, "fp64")
+    with pytest.raises(_lib.FasesError) as e:
+        sv.set_receivers([10 ** 9])
+    assert e.value.code == _lib.FASES_E_RANGE
+    with pytest.raises(_lib.FasesError) as e:
+        sv.run(10 ** 6)      # waveform shorter than the requested steps
+    assert e.value.code == _lib.FASES_E_ARG
+    bad = Materials.rigid(2)
+    with pytest.raises(_lib.FasesError):
+        sv.set_materials(bad)   # the room's oid refers to 6 materials
