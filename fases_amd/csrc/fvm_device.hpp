@@ -23,6 +23,14 @@ template <> struct Conv<float, float> {
     __device__ static inline float ld(float x, float) { return x; }
     __device__ static inline float st(float x, float) { return x; }
 };
+template <> struct Conv<float, double> {
+    __device__ static inline double ld(float x, double) { return (double)x; }
+    __device__ static inline float st(double x, double) { return (float)x; }
+};
+template <> struct Conv<__half, double> {
+    __device__ static inline double ld(__half x, double scale) { return (double)__half2float(x) * scale; }
+    __device__ static inline __half st(double x, double inv_scale) { return __float2half_rn((float)(x * inv_scale)); }
+};
 template <> struct Conv<__half, float> {
     __device__ static inline float ld(__half x, float scale) { return __half2float(x) * scale; }
     __device__ static inline __half st(float x, float inv_scale) { return __float2half_rn(x * inv_scale); }

@@ -98,6 +98,7 @@ class Workload:
     src_sval: list = field(default_factory=list)
     src_opt: int = _lib.FASES_SRC_RICKER
     u_s: Optional[np.ndarray] = None
+    src_freq: Optional[list] = None    # per-source frequency of sine sources [Hz]
     ic: Optional[dict] = None          # {"center": (i,j,k), "sigma": cells, "amp": Pa}
     _scene: Optional[Scene] = None
 
@@ -222,7 +223,7 @@ def make_solver(wl: Workload, precision: Optional[str] = None):
     sv.set_geometry(sc.S, sc.oid, sc.dims)
     sv.set_materials(wl.materials)
     if wl.src_ijk:
-        sv.set_sources([wl.cell(s) for s in wl.src_ijk], wl.src_sval, wl.src_opt, wl.u_s)
+        sv.set_sources([wl.cell(s) for s in wl.src_ijk], wl.src_sval, wl.src_opt, wl.u_s, wl.src_freq)
     sv.set_receivers([wl.cell(r) for r in wl.rec_ijk])
     p0 = wl.initial_pressure()
     sv.set_fields(p0)

@@ -94,7 +94,7 @@ def run_workload(wl, nsteps=None, want_fields=False, want_vel=False):
     sc = wl.scene
     return run(sc.S, sc.oid, sc.dims, wl.params, wl.materials, n,
                src_cells=[wl.cell(s) for s in wl.src_ijk], sval=wl.src_sval, src_opt=wl.src_opt, u_s=wl.u_s,
-               rec_cells=[wl.cell(r) for r in wl.rec_ijk], ic=(wl.initial_pressure(), None, None, None),
+               freq=wl.src_freq, rec_cells=[wl.cell(r) for r in wl.rec_ijk], ic=(wl.initial_pressure(), None, None, None),
                want_fields=want_fields, want_vel=want_vel)
 
 

@@ -100,29 +100,24 @@ def test_fp32_rigid_box():
 
 
 @pytest.mark.parametrize("opt", [_lib.FASES_SRC_PCS, _lib.FASES_SRC_SINE])
-def test_fp64_source_options(opt):
+@pytest.mark.parametrize("prec,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
+def test_source_options(opt, prec, tol):
     wl = small_ade(seed=5, room=(28, 24, 20), nsteps=150, n_rec=2)
     if opt == _lib.FASES_SRC_PCS:
         u_s = W.source_waveform(opt, wl.params.dt, 25.0, 1400.0, wl.nsteps)
         wl = dataclasses.replace(wl, src_opt=opt, u_s=u_s, src_sval=[3.0e4])
-        freq = None
     else:
-        wl = dataclasses.replace(wl, src_opt=opt, u_s=None, src_sval=[2.0])
-        freq = [900.0]
-    announce("fp64 source option %d" % opt)
-    sv = W.make_solver(wl, "fp64")
-    sv.set_sources([wl.cell(s) for s in wl.src_ijk], wl.src_sval, opt, wl.u_s, freq)
-    ir, _, _, _ = sv.run(wl.nsteps)
-    sc = wl.scene
-    ref = O.run(sc.S, sc.oid, sc.dims, wl.params, wl.materials, wl.nsteps, src_cells=[wl.cell(s) for s in wl.src_ijk],
-                sval=wl.src_sval, src_opt=opt, u_s=wl.u_s, freq=freq, rec_cells=[wl.cell(r) for r in wl.rec_ijk])
+        wl = dataclasses.replace(wl, src_opt=opt, u_s=None, src_sval=[2.0], src_freq=[900.0])
+    announce("%s source option %d" % (prec, opt))
+    ir, _, _ = gpu_run(wl, prec, want_vel=False)
+    ref = O.run_workload(wl)
     assert np.abs(ref["ir"]).max() > 0
-    assert rel_l2(ir, ref["ir"]) <= FP64_TOL
+    assert rel_l2(ir, ref["ir"]) <= tol
 
 
 def test_run_chaining_is_exact():
     """run(a) + run(b) continues the same trajectory as run(a+b), bit for bit."""
-    wl = small_ade(seed=9, room=(24, 20, 18), nsteps=120, n_rec=2)
+    wl = small_ade(seed=9, room=(36, 20, 18), nsteps=120, n_rec=2)
     announce("chaining")
     sv = W.make_solver(wl, "fp64")
     a, _, _, _ = sv.run(50)
@@ -134,7 +129,7 @@ def test_run_chaining_is_exact():
 
 def test_same_cell_sources_merge():
     """Two sources in one cell act as one source with the summed drive."""
-    wl = small_ade(seed=2, room=(24, 20, 18), nsteps=80, n_rec=2)
+    wl = small_ade(seed=2, room=(24, 20, 18), nsteps=80, n_rec=2, pml_x_hi=False)
     announce("same-cell sources")
     sv = W.make_solver(wl, "fp64")
     c = wl.cell(wl.src_ijk[0])
@@ -147,7 +142,7 @@ def test_same_cell_sources_merge():
 
 
 def test_error_paths_on_device():
-    wl = small_ade(seed=1, room=(16, 16, 16), nsteps=10, n_rec=1)
+    wl = small_ade(seed=1, room=(18, 18, 18), nsteps=10, n_rec=1, pml_x_hi=False)
     announce("error paths")
     sv = W.make_solver(wl, "fp64")
     with pytest.raises(_lib.FasesError) as e:
