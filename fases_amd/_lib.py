@@ -42,6 +42,7 @@ class fases_params(C.Structure):
         ("pml_alpha_max", C.c_double),
         ("fp16_rescale_every", C.c_int),
         ("use_graph", C.c_int),
+        ("kernel_variant", C.c_int),
     ]
 
 

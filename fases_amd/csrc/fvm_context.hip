@@ -11,6 +11,7 @@
 // profiles are 1-D tables per axis.
 
 #include "fvm_device.hpp"
+#include "fvm_tiled.hpp"
 #include "host.hpp"
 #include "../../include/fases_core.h"
 
@@ -44,20 +45,6 @@ constexpr int kSourcePatch = 12;  // FP64 shadow radius around point sources (re
 
 template <typename S> struct FieldSet {
     S *p, *u, *v, *w;
-};
-
-struct PmlAxis {           // per-axis 1-D cPML tables, length = grid extent on that axis
-    const int *band;       // -1 outside the band, else tile index in [0, 2d)
-    const double *kc;      // 1/kappa - 1
-    const double *b;       // exp(-(alpha + sigma/kappa) dt)
-    const double *a;       // sigma/(kappa(sigma + kappa alpha)) (b - 1), 0 where sigma == 0
-};
-
-struct PmlSet {
-    int on;                // any face enabled
-    int d;                 // depth in cells
-    PmlAxis x, y, z;
-    double *psi_px, *psi_py, *psi_pz, *psi_ux, *psi_vy, *psi_wz;
 };
 
 struct Grid {
@@ -112,6 +99,7 @@ template <typename S> struct PatchIO {
 template <typename IO, typename R, int STAGE>
 __device__ inline void flux_cell(const Grid &g, int i, int j, int k, const IO &io, const PmlSet &pml, R rho, R c,
                                  R h, R dt) {
+#pragma clang fp contract(off)
     const long long C = (long long)i + (long long)g.Nx * j + g.nxy * k;
 
     // fluxes through the faces C+1/2 (P) and C-1/2 (M) along one axis
@@ -161,48 +149,9 @@ __device__ inline void flux_cell(const Grid &g, int i, int j, int k, const IO &i
             wgt = R(2) / R(3);
         }
     }
-    if (pml.on) {
-        // cPML (functions3D.cuh:681-756): stretch with the frozen psi at this stage's RK
-        // weight; advance psi once per step in the last stage with that stage's divergence.
-        const R wdt = wgt * dt;
-        const int bx = pml.x.band[i], by = pml.y.band[j], bz = pml.z.band[k];
-        if (bx >= 0) {
-            const long long X = bx + (long long)(2 * pml.d) * (j + (long long)g.Ny * k);
-            const R Lp = (-FpP + FpM) / h, Lu = (-FuP + FuM) / h;
-            const R kc = (R)pml.x.kc[i];
-            po = po + wdt * (kc * Lp + (R)pml.psi_px[X]);
-            uo = uo + wdt * (kc * Lu + (R)pml.psi_ux[X]);
-            if (STAGE == 2) {
-                const R b = (R)pml.x.b[i], a = (R)pml.x.a[i];
-                pml.psi_px[X] = (double)(b * (R)pml.psi_px[X] + a * Lp);
-                pml.psi_ux[X] = (double)(b * (R)pml.psi_ux[X] + a * Lu);
-            }
-        }
-        if (by >= 0) {
-            const long long Y = i + (long long)g.Nx * (by + (long long)(2 * pml.d) * k);
-            const R Lp = (-GpP + GpM) / h, Lv = (-GvP + GvM) / h;
-            const R kc = (R)pml.y.kc[j];
-            po = po + wdt * (kc * Lp + (R)pml.psi_py[Y]);
-            vo = vo + wdt * (kc * Lv + (R)pml.psi_vy[Y]);
-            if (STAGE == 2) {
-                const R b = (R)pml.y.b[j], a = (R)pml.y.a[j];
-                pml.psi_py[Y] = (double)(b * (R)pml.psi_py[Y] + a * Lp);
-                pml.psi_vy[Y] = (double)(b * (R)pml.psi_vy[Y] + a * Lv);
-            }
-        }
-        if (bz >= 0) {
-            const long long Z = i + (long long)g.Nx * (j + (long long)g.Ny * bz);
-            const R Lp = (-HpP + HpM) / h, Lw = (-HwP + HwM) / h;
-            const R kc = (R)pml.z.kc[k];
-            po = po + wdt * (kc * Lp + (R)pml.psi_pz[Z]);
-            wo = wo + wdt * (kc * Lw + (R)pml.psi_wz[Z]);
-            if (STAGE == 2) {
-                const R b = (R)pml.z.b[k], a = (R)pml.z.a[k];
-                pml.psi_pz[Z] = (double)(b * (R)pml.psi_pz[Z] + a * Lp);
-                pml.psi_wz[Z] = (double)(b * (R)pml.psi_wz[Z] + a * Lw);
-            }
-        }
-    }
+    if (pml.on)
+        pml_apply<R, STAGE == 2>(pml, i, j, k, g.Nx, g.Ny, po, uo, vo, wo, (-FpP + FpM) / h, (-GpP + GpM) / h,
+                                 (-HpP + HpM) / h, (-FuP + FuM) / h, (-GvP + GvM) / h, (-HwP + HwM) / h, wgt * dt);
     io.st(1, C, uo);
     io.st(2, C, vo);
     io.st(3, C, wo);
@@ -253,13 +202,15 @@ __global__ void k_patch_gather(FieldSet<S> f, const int *__restrict__ cells, int
 // ------------------------------------------------------------------------------------
 template <typename S, typename R>
 __global__ void k_sources(S *__restrict__ p, const int *__restrict__ cell, const double *__restrict__ sval,
-                          const double *__restrict__ freq, const double *__restrict__ u_s, int n_src, int opt,
-                          long long nn, double w, double dt, double c, double rho, double As, double h,
+                          const double *__restrict__ freq, const double *__restrict__ u_s, long long us_stride,
+                          int n_src, int opt, long long nn, double w, double dt, double c, double rho, double As, double h,
                           const int *__restrict__ pidx, double *__restrict__ sh_p, R scale, R inv_scale) {
+#pragma clang fp contract(off)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_src) return;
     typedef Conv<S, R> CV;
-    const double amp = (opt == FASES_SRC_SINE) ? sin(2.0 * M_PI * freq[s] * (double)nn * (0.5 * dt)) : u_s[nn];
+    (void)freq;
+    const double amp = u_s[(long long)s * us_stride + nn];   // shared waveform (stride 0) or per-source table
     const double add = w * dt * c * c * sval[s] * amp * rho * As / (h * h * h);
     const int C = cell[s];
     const int q = pidx ? pidx[C] : -1;
@@ -296,6 +247,7 @@ struct Ade {
 template <typename S>
 __device__ inline double tri_probe(const S *__restrict__ f, int base, int sx_, double sx, double sy, double sz,
                                    int nx, long long nxy, double scale) {
+#pragma clang fp contract(off)
     typedef Conv<S, typename std::conditional<std::is_same<S, double>::value, double, float>::type> CV;
     (void)sx_;
     const double f000 = (double)CV::ld(f[base], scale);
@@ -321,6 +273,7 @@ k_boundary(Grid g, FieldSet<S> in, FieldSet<S> out, Bnd bd, Ade ade, double rho,
            const double *__restrict__ phi_r, double *__restrict__ p1_l, const double *__restrict__ p1_m,
            const double *__restrict__ p1_r, double *__restrict__ p2_l, const double *__restrict__ p2_m,
            const double *__restrict__ p2_r, double scale_in, double scale_out, double inv_scale_out) {
+#pragma clang fp contract(off)
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= bd.n) return;
     typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type R;
@@ -442,6 +395,7 @@ struct fases_ctx {
     long long n_samp = 0;
     int *d_scell = nullptr;
     double *d_sval = nullptr, *d_sfreq = nullptr, *d_us = nullptr;
+    std::vector<double> h_sfreq;         // sine sources: frequencies (samples tabulated per run)
     int n_rec = 0;
     int *d_rcell = nullptr;
     double *d_ir = nullptr, *d_vel = nullptr;
@@ -565,16 +519,36 @@ int launch_step(fases_ctx *ctx, long long n, std::vector<hipEvent_t> *kev) {
             kev->push_back(e1);
             HIPCHK(hipEventRecord(e0, ctx->stream));
         }
-        const GridIO<S, R> gio{fset<S>(ctx, q.in), fset<S>(ctx, 0), fset<S>(ctx, q.out), one, one, one};
-        if (s == 0)
-            hipLaunchKernelGGL((k_flux_simple<S, R, 0>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml, rho,
-                               c, h, dt);
-        else if (s == 1)
-            hipLaunchKernelGGL((k_flux_simple<S, R, 1>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml, rho,
-                               c, h, dt);
-        else
-            hipLaunchKernelGGL((k_flux_simple<S, R, 2>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml, rho,
-                               c, h, dt);
+        const int kv = ctx->prm.kernel_variant;
+        if (kv == 1 || (kv == 0 && ctx->prm.precision == FASES_FP64)) {
+            const GridIO<S, R> gio{fset<S>(ctx, q.in), fset<S>(ctx, 0), fset<S>(ctx, q.out), one, one, one};
+            if (s == 0)
+                hipLaunchKernelGGL((k_flux_simple<S, R, 0>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml,
+                                   rho, c, h, dt);
+            else if (s == 1)
+                hipLaunchKernelGGL((k_flux_simple<S, R, 1>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml,
+                                   rho, c, h, dt);
+            else
+                hipLaunchKernelGGL((k_flux_simple<S, R, 2>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml,
+                                   rho, c, h, dt);
+        } else {
+            const FieldSet<S> fi = fset<S>(ctx, q.in), fb = fset<S>(ctx, 0), fo = fset<S>(ctx, q.out);
+            const dim3 tb(64, tile::NW);
+            const dim3 tg((ctx->Nx - 6 + tile::TXO - 1) / tile::TXO, (ctx->Ny - 6 + tile::TYO - 1) / tile::TYO,
+                          (ctx->Nz - 6 + tile::TZ - 1) / tile::TZ);
+            if (s == 0)
+                hipLaunchKernelGGL((k_flux_tiled<S, R, 0>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, fi.p,
+                                   fi.u, fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w, ctx->d_upd,
+                                   ctx->pml, rho, c, h, dt, one, one, one);
+            else if (s == 1)
+                hipLaunchKernelGGL((k_flux_tiled<S, R, 1>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, fi.p,
+                                   fi.u, fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w, ctx->d_upd,
+                                   ctx->pml, rho, c, h, dt, one, one, one);
+            else
+                hipLaunchKernelGGL((k_flux_tiled<S, R, 2>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, fi.p,
+                                   fi.u, fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w, ctx->d_upd,
+                                   ctx->pml, rho, c, h, dt, one, one, one);
+        }
         if (kev) HIPCHK(hipEventRecord(e1, ctx->stream));
         if (ctx->n_patch > 0) {
             const dim3 pg((ctx->n_patch + 63) / 64), pb(64);
@@ -592,8 +566,8 @@ int launch_step(fases_ctx *ctx, long long n, std::vector<hipEvent_t> *kev) {
         }
         if (ctx->n_src > 0)
             hipLaunchKernelGGL((k_sources<S, R>), dim3((ctx->n_src + sb - 1) / sb), dim3(sb), 0, ctx->stream,
-                               (S *)ctx->U[q.out][0], ctx->d_scell, ctx->d_sval, ctx->d_sfreq, ctx->d_us, ctx->n_src,
-                               ctx->src_opt, q.nn, q.w, P.dt, P.c, P.rho, As, P.h,
+                               (S *)ctx->U[q.out][0], ctx->d_scell, ctx->d_sval, ctx->d_sfreq, ctx->d_us,
+                               ctx->src_opt == FASES_SRC_SINE ? ctx->n_samp : 0LL, ctx->n_src, ctx->src_opt, q.nn, q.w, P.dt, P.c, P.rho, As, P.h,
                                ctx->n_patch > 0 ? ctx->d_pidx : nullptr, ctx->shadow[q.out], one, one);
         if (ctx->nB > 0)
             hipLaunchKernelGGL((k_boundary<S>), dim3((ctx->nB + sb - 1) / sb), dim3(sb), 0, ctx->stream, g,
@@ -623,6 +597,18 @@ int run_impl(fases_ctx *ctx, long long nsteps, double *ir, double *vel, double *
     const long long need = (long long)ctx->n_rec * nsteps;
     if (ctx->n_src > 0 && ctx->src_opt != FASES_SRC_SINE && 2 * (ctx->step + nsteps) + 1 > ctx->n_samp)
         return set_error(ctx, FASES_E_ARG, "source waveform shorter than 2*(steps)+1 samples");
+    if (ctx->n_src > 0 && ctx->src_opt == FASES_SRC_SINE && 2 * (ctx->step + nsteps) + 1 > ctx->n_samp) {
+        // per-source sine samples sin(2 pi f nn dt/2), tabulated on the host (same libm
+        // as a CPU evaluation of the reference formula)
+        const long long ns = 2 * (ctx->step + nsteps) + 1;
+        std::vector<double> tab((size_t)ctx->n_src * ns);
+        for (int q = 0; q < ctx->n_src; ++q)
+            for (long long t = 0; t < ns; ++t)
+                tab[(size_t)q * ns + t] = std::sin(2.0 * M_PI * ctx->h_sfreq[(size_t)q] * (double)t * (0.5 * ctx->prm.dt));
+        int rc = upload(ctx, &ctx->d_us, tab.data(), tab.size());
+        if (rc) return rc;
+        ctx->n_samp = ns;
+    }
     if (ctx->n_rec > 0 && nsteps > ctx->ir_cap) {
         // grow the device receiver buffers
         int rc = dalloc(ctx, &ctx->d_ir, (size_t)need);
@@ -952,6 +938,7 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
     ctx->n_src = (int)mc.size();
     ctx->src_opt = src_opt;
     ctx->n_samp = (src_opt == FASES_SRC_SINE) ? 0 : n_samp;
+    ctx->h_sfreq = mf;
     if ((rc = upload(ctx, &ctx->d_scell, mc.data(), mc.size()))) return rc;
     if ((rc = upload(ctx, &ctx->d_sval, ms.data(), ms.size()))) return rc;
     if ((rc = upload(ctx, &ctx->d_sfreq, mf.data(), mf.size()))) return rc;

@@ -43,8 +43,11 @@ template <> struct Conv<__half, float> {
 // Candidate rows c[rr] = {11/6,-7/6,1/3}, {1/3,5/6,-1/6}, {-1/6,5/6,1/3}, {1/3,-7/6,11/6}
 // (main3D.cu:834-837), beta rows {3,-4,1}, {1,0,-1}, {1,-4,3} (:839-841), linear
 // weights d = {3/10, 3/5, 1/10} (:843) mirrored for the right state.
+// The literal-reference functions below evaluate without FMA contraction so that the
+// FP64 per-cell kernel rounds exactly like the FP64 CPU restatement of the reference.
 template <typename R>
 __device__ inline R weno_smooth(R x, R y, R z, R k0, R k1, R k2) {
+#pragma clang fp contract(off)
     const R s = x - R(2) * y + z;
     const R t = k0 * x + k1 * y + k2 * z;
     return (R(13) * s * s / R(12)) + (R(0.25) * t * t);
@@ -52,6 +55,7 @@ __device__ inline R weno_smooth(R x, R y, R z, R k0, R k1, R k2) {
 
 template <typename R>
 __device__ inline R weno_combine(R v0, R v1, R v2, R b0, R b1, R b2, R d0, R d1, R d2) {
+#pragma clang fp contract(off)
     const R eps = R(1e-10);
     const R tau = fabs(b0 - b2);
     const R a0 = d0 * (R(1) + tau / (b0 + eps));
@@ -67,6 +71,7 @@ __device__ inline R weno_combine(R v0, R v1, R v2, R b0, R b1, R b2, R d0, R d1,
 
 template <typename R>
 __device__ inline void weno_face(R a, R b, R c, R d, R e, R f, R &left, R &right) {
+#pragma clang fp contract(off)
     const R c3 = R(1) / R(3), c56 = R(5) / R(6), c16 = R(1) / R(6), c116 = R(11) / R(6), c76 = R(7) / R(6);
     // cell i (centre c): candidates r = 0,1,2 on (c,d,e), (b,c,d), (a,b,c)
     {
@@ -94,6 +99,7 @@ __device__ inline void weno_face(R a, R b, R c, R d, R e, R f, R &left, R &right
 // with left state (pL,uL) and right state (pR,uR); u = the face-normal velocity.
 template <typename R>
 __device__ inline void godunov(R pL, R pR, R uL, R uR, R rho, R c, R &Fp, R &Fu) {
+#pragma clang fp contract(off)
     Fp = R(0.5) * (rho * c * c * (uR + uL) - c * (pR - pL));
     Fu = R(0.5) * ((pR + pL) / rho - c * (uR - uL));
 }
@@ -105,6 +111,65 @@ __host__ __device__ inline R chi_band(R s, R h, R tol) {
     if (s <= h) return R(1);
     const R r = (s - h) / (tol - h);
     return R(1) - r * r * (R(3) - R(2) * r);
+}
+
+// cPML of one updated cell (functions3D.cuh:681-756): the frozen psi stretches the
+// stage divergence at the stage's RK weight (wdt = weight * dt); the last stage
+// (ADVANCE) then advances psi once per step with its own divergence. Per-axis 1-D
+// tables hold band index, 1/kappa - 1, b = exp(-(alpha + sigma/kappa) dt) and
+// a = sigma/(kappa(sigma + kappa alpha)) (b - 1).
+struct PmlAxis {
+    const int *band;       // -1 outside the band, else tile index in [0, 2d)
+    const double *kc;
+    const double *b;
+    const double *a;
+};
+
+struct PmlSet {
+    int on;
+    int d;
+    PmlAxis x, y, z;
+    double *psi_px, *psi_py, *psi_pz, *psi_ux, *psi_vy, *psi_wz;
+};
+
+template <typename R, bool ADVANCE>
+__device__ inline void pml_apply(const PmlSet &pml, int i, int j, int k, int Nx, int Ny, R &po, R &uo, R &vo, R &wo,
+                                 R Lpx, R Lpy, R Lpz, R Lux, R Lvy, R Lwz, R wdt) {
+#pragma clang fp contract(off)
+    const int bx = pml.x.band[i], by = pml.y.band[j], bz = pml.z.band[k];
+    if (bx >= 0) {
+        const long long X = bx + (long long)(2 * pml.d) * (j + (long long)Ny * k);
+        const R kc = (R)pml.x.kc[i];
+        po = po + wdt * (kc * Lpx + (R)pml.psi_px[X]);
+        uo = uo + wdt * (kc * Lux + (R)pml.psi_ux[X]);
+        if (ADVANCE) {
+            const R b = (R)pml.x.b[i], a = (R)pml.x.a[i];
+            pml.psi_px[X] = (double)(b * (R)pml.psi_px[X] + a * Lpx);
+            pml.psi_ux[X] = (double)(b * (R)pml.psi_ux[X] + a * Lux);
+        }
+    }
+    if (by >= 0) {
+        const long long Y = i + (long long)Nx * (by + (long long)(2 * pml.d) * k);
+        const R kc = (R)pml.y.kc[j];
+        po = po + wdt * (kc * Lpy + (R)pml.psi_py[Y]);
+        vo = vo + wdt * (kc * Lvy + (R)pml.psi_vy[Y]);
+        if (ADVANCE) {
+            const R b = (R)pml.y.b[j], a = (R)pml.y.a[j];
+            pml.psi_py[Y] = (double)(b * (R)pml.psi_py[Y] + a * Lpy);
+            pml.psi_vy[Y] = (double)(b * (R)pml.psi_vy[Y] + a * Lvy);
+        }
+    }
+    if (bz >= 0) {
+        const long long Z = i + (long long)Nx * (j + (long long)Ny * bz);
+        const R kc = (R)pml.z.kc[k];
+        po = po + wdt * (kc * Lpz + (R)pml.psi_pz[Z]);
+        wo = wo + wdt * (kc * Lwz + (R)pml.psi_wz[Z]);
+        if (ADVANCE) {
+            const R b = (R)pml.z.b[k], a = (R)pml.z.a[k];
+            pml.psi_pz[Z] = (double)(b * (R)pml.psi_pz[Z] + a * Lpz);
+            pml.psi_wz[Z] = (double)(b * (R)pml.psi_wz[Z] + a * Lwz);
+        }
+    }
 }
 
 }  // namespace fasesdev

@@ -55,6 +55,7 @@ int fases_params_default(fases_params *p) {
     p->pml_alpha_max = M_PI * 20.0;
     p->fp16_rescale_every = 100;
     p->use_graph = 0;
+    p->kernel_variant = 0;
     return FASES_OK;
 }
 

@@ -252,6 +252,7 @@ class SolverParams:
     pml_alpha_max: float = float(np.pi * 20.0)
     fp16_rescale_every: int = 100
     use_graph: bool = False
+    kernel_variant: int = 0             # 0 auto, 1 per-cell (reference order), 2 tiled
 
     def to_c(self) -> _lib.fases_params:
         p = _lib.fases_params()
@@ -267,6 +268,7 @@ class SolverParams:
         p.pml_alpha_max = self.pml_alpha_max
         p.fp16_rescale_every = self.fp16_rescale_every
         p.use_graph = int(self.use_graph)
+        p.kernel_variant = int(self.kernel_variant)
         return p
 
 

@@ -67,6 +67,10 @@ typedef struct fases_params {
     double pml_alpha_max;   /* reference 20 pi rad/s                                   */
     int    fp16_rescale_every; /* FP16: steps between per-field scale updates (100)    */
     int    use_graph;       /* 1: replay the step loop as a captured hipGraph          */
+    int    kernel_variant;  /* 0 auto: FP64 -> per-cell kernel, FP32/FP16 -> tiled;
+                               1 per-cell kernel (reference operation order, no FMA
+                               contraction: bit-faithful to an FP64 CPU evaluation);
+                               2 tiled kernel (faces reconstructed once, LDS/shuffles) */
 } fases_params;
 
 /* Fills *p with the reference defaults (FP64, device 0, PML depth 20, ...). */

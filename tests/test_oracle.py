@@ -186,3 +186,16 @@ def test_rigid_box_interior_energy_decays():
     assert max(Ei) <= E0
     assert Ei[-1] < 0.6 * E0
     assert np.mean(Ea[-10:]) <= np.mean(Ea[:10])
+
+
+@pytest.mark.slow
+def test_point_source_sensitivity():
+    """Documents the property that sets the FP64 parity strategy: near a point source the
+    WENO-Z scheme amplifies a 2e-15 relative change of the source amplitude to > 1e-8 in
+    the pressure field within 40 steps (the same room as the GPU bitwise test)."""
+    import dataclasses
+    wl = W.ade_room(room=(130, 33, 150), h=0.02, nsteps=40, seed=21, n_rec=1, pml_cells=6)
+    a = O.run_workload(wl, want_fields=True)["fields"][0]
+    wl2 = dataclasses.replace(wl, src_sval=[wl.src_sval[0] * (1 + 2e-15)])
+    b = O.run_workload(wl2, want_fields=True)["fields"][0]
+    assert np.linalg.norm(a - b) / np.linalg.norm(a) > 1e-8
