@@ -230,12 +230,12 @@ __global__ void k_sources(S *__restrict__ p, const int *__restrict__ cell, const
 // q = (1-chi) q + chi q_b acts on the stage OUTPUT. All arithmetic is FP64.
 // ADE stage (a,b,c): x_left = (a x_mid + b (x_right + dt xdot(x_right))) / c.
 // ------------------------------------------------------------------------------------
-struct Bnd {
+template <typename G> struct Bnd {
     int n;
     const int *cell;        // grid index of the boundary cell
     const int *base;        // grid index of the trilinear base corner of the probe point
-    const int *mat;         // material index (oid - 1)
-    const double *geo;      // [8*n]: sx, sy, sz, nx, ny, nz, chi, pad
+    const int *mat;         // bits 0-7: material index (oid - 1); bit 8: probe on a cell centre
+    const G *geo;           // [8*n]: sx, sy, sz, nx, ny, nz, chi, pad (FP64 for FP64 fields)
 };
 
 struct Ade {
@@ -266,9 +266,9 @@ __device__ inline double tri_probe(const S *__restrict__ f, int base, int sx_, d
     return c0 * (1 - sz) + c1 * sz;
 }
 
-template <typename S>
+template <typename S, typename G>
 __global__ void __launch_bounds__(256)
-k_boundary(Grid g, FieldSet<S> in, FieldSet<S> out, Bnd bd, Ade ade, double rho, double c, double dt,
+k_boundary(Grid g, FieldSet<S> in, FieldSet<S> out, Bnd<G> bd, Ade ade, double rho, double c, double dt,
            double as, double bs, double cs, double *__restrict__ phi_l, const double *__restrict__ phi_m,
            const double *__restrict__ phi_r, double *__restrict__ p1_l, const double *__restrict__ p1_m,
            const double *__restrict__ p1_r, double *__restrict__ p2_l, const double *__restrict__ p2_m,
@@ -278,13 +278,24 @@ k_boundary(Grid g, FieldSet<S> in, FieldSet<S> out, Bnd bd, Ade ade, double rho,
     if (b >= bd.n) return;
     typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type R;
     typedef Conv<S, R> CV;
-    const int C = bd.cell[b], base = bd.base[b], m = bd.mat[b];
-    const double *q = bd.geo + 8 * (long long)b;
+    const int C = bd.cell[b], base = bd.base[b], mw = bd.mat[b];
+    const int m = mw & 0xFF;
+    const G *q = bd.geo + 8 * (long long)b;
     const double sx = q[0], sy = q[1], sz = q[2], nx = q[3], ny = q[4], nz = q[5], chi = q[6];
-    const double pD = tri_probe(in.p, base, 0, sx, sy, sz, g.Nx, g.nxy, scale_in);
-    const double uD = tri_probe(in.u, base, 0, sx, sy, sz, g.Nx, g.nxy, scale_in);
-    const double vD = tri_probe(in.v, base, 0, sx, sy, sz, g.Nx, g.nxy, scale_in);
-    const double wD = tri_probe(in.w, base, 0, sx, sy, sz, g.Nx, g.nxy, scale_in);
+    double pD, uD, vD, wD;
+    if (mw & 0x100) {
+        // probe exactly on a cell centre (all trilinear fractions 0): the interpolant is
+        // that cell's value, bit for bit
+        pD = (double)CV::ld(in.p[base], (R)scale_in);
+        uD = (double)CV::ld(in.u[base], (R)scale_in);
+        vD = (double)CV::ld(in.v[base], (R)scale_in);
+        wD = (double)CV::ld(in.w[base], (R)scale_in);
+    } else {
+        pD = tri_probe(in.p, base, 0, sx, sy, sz, g.Nx, g.nxy, scale_in);
+        uD = tri_probe(in.u, base, 0, sx, sy, sz, g.Nx, g.nxy, scale_in);
+        vD = tri_probe(in.v, base, 0, sx, sy, sz, g.Nx, g.nxy, scale_in);
+        wD = tri_probe(in.w, base, 0, sx, sy, sz, g.Nx, g.nxy, scale_in);
+    }
     const double un = uD * nx + vD * ny + wD * nz;
     const double utx = uD - un * nx, uty = vD - un * ny, utz = wD - un * nz;
     const double Z0 = rho * c;
@@ -489,8 +500,51 @@ FieldSet<S> fset(fases_ctx *ctx, int s) {
     return FieldSet<S>{(S *)ctx->U[s][0], (S *)ctx->U[s][1], (S *)ctx->U[s][2], (S *)ctx->U[s][3]};
 }
 
+template <typename R>
+Split<R> split_const(double v) {
+    const R hi = (R)v;
+    return Split<R>{hi, (R)(v - (double)hi)};
+}
+
+template <typename R>
+TileK<R> tile_constants(const fases_params &P) {
+    TileK<R> K;
+    K.kp = split_const<R>(0.5 * P.rho * P.c * P.c);
+    K.kr = split_const<R>(0.5 / P.rho);
+    K.dth = split_const<R>(P.dt / P.h);
+    K.third = split_const<R>(1.0 / 3.0);
+    K.kc = (R)(0.5 * P.c);
+    K.ih = (R)(1.0 / P.h);
+    return K;
+}
+
+template <typename S, typename R, int NW>
+int launch_tiled(fases_ctx *ctx, int stage, int in, int out, R rho, R c, R h, R dt) {
+    (void)rho; (void)c; (void)h;
+    const TileK<R> K = tile_constants<R>(ctx->prm);
+    const FieldSet<S> fi = fset<S>(ctx, in), fb = fset<S>(ctx, 0), fo = fset<S>(ctx, out);
+    const dim3 tb(64, NW);
+    const dim3 tg((ctx->Nx - 6 + tile::TXO - 1) / tile::TXO, (ctx->Ny - 6 + (NW - 2) - 1) / (NW - 2),
+                  (ctx->Nz - 6 + tile::TZ - 1) / tile::TZ);
+    const R one = R(1);
+    if (stage == 0)
+        hipLaunchKernelGGL((k_flux_tiled<S, R, 0, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, fi.p, fi.u,
+                           fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w, ctx->d_upd, ctx->pml, K, dt,
+                           one, one, one);
+    else if (stage == 1)
+        hipLaunchKernelGGL((k_flux_tiled<S, R, 1, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, fi.p, fi.u,
+                           fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w, ctx->d_upd, ctx->pml, K, dt,
+                           one, one, one);
+    else
+        hipLaunchKernelGGL((k_flux_tiled<S, R, 2, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, fi.p, fi.u,
+                           fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w, ctx->d_upd, ctx->pml, K, dt,
+                           one, one, one);
+    return FASES_OK;
+}
+
 template <typename S, typename R>
 int launch_step(fases_ctx *ctx, long long n, std::vector<hipEvent_t> *kev) {
+    int rc = FASES_OK;
     Grid g{ctx->Nx, ctx->Ny, ctx->Nz, (long long)ctx->Nx * ctx->Ny};
     const fases_params &P = ctx->prm;
     const R rho = (R)P.rho, c = (R)P.c, h = (R)P.h, dt = (R)P.dt;
@@ -501,7 +555,10 @@ int launch_step(fases_ctx *ctx, long long n, std::vector<hipEvent_t> *kev) {
     const int sb = 256;
     Ade ade{ctx->K, ctx->L, ctx->d_Rinf, ctx->d_A, ctx->d_zeta, ctx->d_B, ctx->d_C, ctx->d_alpha, ctx->d_beta,
             ctx->d_Keff, ctx->d_Leff};
-    Bnd bd{ctx->nB, ctx->d_bcell, ctx->d_bbase, ctx->d_bmat, ctx->d_bgeo};
+    // Geometry stays FP64 for every field precision: a blend weight or normal rounded
+    // once to FP32 is a bias that every step repeats (measured: FP32-room IR error x3.5).
+    typedef double G;
+    Bnd<G> bd{ctx->nB, ctx->d_bcell, ctx->d_bbase, ctx->d_bmat, ctx->d_bgeo};
     // (field set in, set out, RK weight, source sample, ADE a,b,c, ADE buffers left/mid/right)
     struct StageCfg { int in, out; double w; long long nn; double a, b, c; int l, mid, r; };
     const StageCfg cfg[3] = {
@@ -532,22 +589,8 @@ int launch_step(fases_ctx *ctx, long long n, std::vector<hipEvent_t> *kev) {
                 hipLaunchKernelGGL((k_flux_simple<S, R, 2>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml,
                                    rho, c, h, dt);
         } else {
-            const FieldSet<S> fi = fset<S>(ctx, q.in), fb = fset<S>(ctx, 0), fo = fset<S>(ctx, q.out);
-            const dim3 tb(64, tile::NW);
-            const dim3 tg((ctx->Nx - 6 + tile::TXO - 1) / tile::TXO, (ctx->Ny - 6 + tile::TYO - 1) / tile::TYO,
-                          (ctx->Nz - 6 + tile::TZ - 1) / tile::TZ);
-            if (s == 0)
-                hipLaunchKernelGGL((k_flux_tiled<S, R, 0>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, fi.p,
-                                   fi.u, fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w, ctx->d_upd,
-                                   ctx->pml, rho, c, h, dt, one, one, one);
-            else if (s == 1)
-                hipLaunchKernelGGL((k_flux_tiled<S, R, 1>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, fi.p,
-                                   fi.u, fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w, ctx->d_upd,
-                                   ctx->pml, rho, c, h, dt, one, one, one);
-            else
-                hipLaunchKernelGGL((k_flux_tiled<S, R, 2>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, fi.p,
-                                   fi.u, fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w, ctx->d_upd,
-                                   ctx->pml, rho, c, h, dt, one, one, one);
+            rc = launch_tiled<S, R, 16>(ctx, s, q.in, q.out, rho, c, h, dt);
+            if (rc) return rc;
         }
         if (kev) HIPCHK(hipEventRecord(e1, ctx->stream));
         if (ctx->n_patch > 0) {
@@ -570,7 +613,7 @@ int launch_step(fases_ctx *ctx, long long n, std::vector<hipEvent_t> *kev) {
                                ctx->src_opt == FASES_SRC_SINE ? ctx->n_samp : 0LL, ctx->n_src, ctx->src_opt, q.nn, q.w, P.dt, P.c, P.rho, As, P.h,
                                ctx->n_patch > 0 ? ctx->d_pidx : nullptr, ctx->shadow[q.out], one, one);
         if (ctx->nB > 0)
-            hipLaunchKernelGGL((k_boundary<S>), dim3((ctx->nB + sb - 1) / sb), dim3(sb), 0, ctx->stream, g,
+            hipLaunchKernelGGL((k_boundary<S, G>), dim3((ctx->nB + sb - 1) / sb), dim3(sb), 0, ctx->stream, g,
                                fset<S>(ctx, q.in), fset<S>(ctx, q.out), bd, ade, P.rho, P.c, P.dt, q.a, q.b, q.c,
                                ctx->phi[q.l], ctx->phi[q.mid], ctx->phi[q.r], ctx->psi1[q.l], ctx->psi1[q.mid],
                                ctx->psi1[q.r], ctx->psi2[q.l], ctx->psi2[q.mid], ctx->psi2[q.r], 1.0, 1.0, 1.0);
@@ -773,10 +816,12 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
         bk = std::max(0, std::min(bk, Nz - 2));
         bcell.push_back((int)C);
         bbase.push_back(bi + Nx * (bj + Ny * bk));
-        bmat.push_back((int)oid[C] - 1);
-        bgeo.push_back(qx - (double)bi);
-        bgeo.push_back(qy - (double)bj);
-        bgeo.push_back(qz - (double)bk);
+        const double fx = qx - (double)bi, fy = qy - (double)bj, fz = qz - (double)bk;
+        const int centred = (fx == 0.0 && fy == 0.0 && fz == 0.0) ? 0x100 : 0;
+        bmat.push_back(((int)oid[C] - 1) | centred);
+        bgeo.push_back(fx);
+        bgeo.push_back(fy);
+        bgeo.push_back(fz);
         bgeo.push_back(nx);
         bgeo.push_back(ny);
         bgeo.push_back(nz);
@@ -837,8 +882,8 @@ int fases_set_materials(fases_ctx *ctx, int nMat, int K, int L, const double *Ri
     if (!ctx->have_geo) return set_error(ctx, FASES_E_STATE, "fases_set_materials: call fases_set_geometry first");
     if (nMat < 1 || K < 1 || L < 1 || !Rinf || !A || !zeta || !B || !C || !alpha || !beta)
         return set_error(ctx, FASES_E_ARG, "fases_set_materials: bad argument");
-    for (int m : ctx->h_bmat)
-        if (m >= nMat) return set_error(ctx, FASES_E_ARG, "fases_set_materials: oid refers to a missing material");
+    for (int mw : ctx->h_bmat)
+        if ((mw & 0xFF) >= nMat) return set_error(ctx, FASES_E_ARG, "fases_set_materials: oid refers to a missing material");
     HIPCHK(hipSetDevice(ctx->prm.device));
     std::vector<int> Ke((size_t)nMat, 0), Le((size_t)nMat, 0);
     for (int m = 0; m < nMat; ++m) {

@@ -38,50 +38,70 @@ __device__ inline void weno_pair(R a, R b, R c, R d, R e, R &A, R &B) {
     const R d1 = b - a, d2 = c - b, d3 = d - c, d4 = e - d;
     const R s0 = d4 - d3, s1 = d3 - d2, s2 = d2 - d1;              // second differences
     const R t0 = fma(R(-3), d3, d4), t1 = d2 + d3, t2 = fma(R(3), d2, -d1);
-    const R k13 = R(13) / R(12);
-    const R b0 = fma(k13 * s0, s0, R(0.25) * (t0 * t0));
-    const R b1 = fma(k13 * s1, s1, R(0.25) * (t1 * t1));
-    const R b2 = fma(k13 * s2, s2, R(0.25) * (t2 * t2));
-    const R eps = R(1e-10);
-    const R tau = fabs(b0 - b2);
-    const R q0 = tau * fast_rcp(b0 + eps), q1 = tau * fast_rcp(b1 + eps), q2 = tau * fast_rcp(b2 + eps);
-    // candidate offsets from c
-    const R c6 = R(1) / R(6), c3 = R(1) / R(3);
-    const R a0 = fma(R(2) * c3, d3, -c6 * d4);        // {1/3,5/6,-1/6}(c,d,e) - c
-    const R a1 = fma(c6, d2, c3 * d3);                // {-1/6,5/6,1/3}(b,c,d) - c
-    const R a2 = fma(R(5) * c6, d2, -c3 * d1);        // {1/3,-7/6,11/6}(a,b,c) - c
-    const R g0 = fma(R(-5) * c6, d3, c3 * d4);        // {11/6,-7/6,1/3}(c,d,e) - c
-    const R g1 = fma(-c3, d2, -c6 * d3);              // {1/3,5/6,-1/6}(b,c,d) - c
-    const R g2 = fma(R(-2) * c3, d2, c6 * d1);        // {-1/6,5/6,1/3}(a,b,c) - c
-    const R al0 = fma(R(0.3), q0, R(0.3)), al1 = fma(R(0.6), q1, R(0.6)), al2 = fma(R(0.1), q2, R(0.1));
-    const R ar0 = fma(R(0.1), q0, R(0.1)), ar1 = al1, ar2 = fma(R(0.3), q2, R(0.3));
-    const R nl = fma(al0, a0, fma(al1, a1, al2 * a2));
-    const R nr = fma(ar0, g0, fma(ar1, g1, ar2 * g2));
-    A = fma(nl, fast_rcp(al0 + al1 + al2), c);
-    B = fma(nr, fast_rcp(ar0 + ar1 + ar2), c);
+    // 4 x smoothness indicators: 13/3 s^2 + t^2 (the weights only see ratios of
+    // beta + eps and |beta_0 - beta_2|, so beta, tau and eps all carry the factor 4)
+    const R k = R(13) / R(3);
+    const R e0 = fma(k * s0, s0, fma(t0, t0, R(4e-10)));
+    const R e1 = fma(k * s1, s1, fma(t1, t1, R(4e-10)));
+    const R e2 = fma(k * s2, s2, fma(t2, t2, R(4e-10)));
+    const R tau = fabs(e0 - e2);
+    // unnormalised WENO-Z weights d_r (1 + tau/e_r) times e0 e1 e2:
+    // g_r = (e_r + tau) * prod_{s != r} e_s
+    const R g0 = (e0 + tau) * (e1 * e2), g1 = (e1 + tau) * (e0 * e2), g2 = (e2 + tau) * (e0 * e1);
+    // candidate offsets from c, pre-multiplied by the linear weights
+    //   left  (V^- at i+1/2): d = 3/10, 3/5, 1/10 on {1/3,5/6,-1/6}(c,d,e), {-1/6,5/6,1/3}(b,c,d),
+    //                          {1/3,-7/6,11/6}(a,b,c)
+    //   right (V^+ at i-1/2): d = 1/10, 3/5, 3/10 on {11/6,-7/6,1/3}(c,d,e), {1/3,5/6,-1/6}(b,c,d),
+    //                          {-1/6,5/6,1/3}(a,b,c)
+    const R a0 = fma(R(0.2), d3, R(-0.05) * d4);
+    const R a1 = fma(R(0.1), d2, R(0.2) * d3);
+    const R a2 = fma(R(1.0 / 12.0), d2, R(-1.0 / 30.0) * d1);
+    const R b0 = fma(R(-1.0 / 12.0), d3, R(1.0 / 30.0) * d4);
+    const R b1 = fma(R(-0.2), d2, R(-0.1) * d3);
+    const R b2 = fma(R(-0.2), d2, R(0.05) * d1);
+    const R nl = fma(g0, a0, fma(g1, a1, g2 * a2));
+    const R nr = fma(g0, b0, fma(g1, b1, g2 * b2));
+    const R dl = fma(R(0.3), g0, fma(R(0.6), g1, R(0.1) * g2));
+    const R dr = fma(R(0.1), g0, fma(R(0.6), g1, R(0.3) * g2));
+    A = fma(nl, fast_rcp(dl), c);
+    B = fma(nr, fast_rcp(dr), c);
 }
 
 namespace tile {
-constexpr int NW = 16;          // waves (rows) per block
-constexpr int TYO = NW - 2;     // output rows per block
 constexpr int TXO = 62;         // output columns per block
 constexpr int LX = 68;          // LDS columns: x0-3 .. x0+64
-constexpr int LY = NW + 4;      // LDS rows:    j0-3 .. j0+NW
 constexpr int TZ = 64;          // planes per block
 }  // namespace tile
 
-template <typename R> struct TileLds {
-    R p[tile::LY][tile::LX];
-    R u[tile::NW][tile::LX];
-    R v[tile::LY][64];
-    R ys[4][tile::NW][64];      // y states: p^-, p^+, v^-, v^+ of every row
+template <typename R, int NW> struct TileLds {
+    R p[NW + 4][tile::LX];      // rows j0-3 .. j0+NW
+    R u[NW][tile::LX];
+    R v[NW + 4][64];
+    R ys[4][NW][64];            // y states: p^-, p^+, v^-, v^+ of every row
 };
 
-// Godunov flux with contraction allowed (tiled kernel).
+// Constants of the tiled kernel as unevaluated sums hi + lo (lo = 0 for FP64): in a
+// lossless resonant room a constant rounded once (dt/h, 1/(2 rho), 1/3) is a systematic
+// bias of the effective sound speed or amplitude that builds up over thousands of steps,
+// so each carries its rounding residual.
+template <typename R> struct Split {
+    R hi, lo;
+};
 template <typename R>
-__device__ inline void godunov_f(R pL, R pR, R uL, R uR, R rho, R c, R &Fp, R &Fu) {
-    Fp = R(0.5) * (rho * c * c * (uR + uL) - c * (pR - pL));
-    Fu = R(0.5) * ((pR + pL) / rho - c * (uR - uL));
+__device__ inline R mul_s(const Split<R> &a, R x) { return fma(a.hi, x, a.lo * x); }
+template <typename R>
+__device__ inline R fma_s(const Split<R> &a, R x, R y) { return fma(a.hi, x, fma(a.lo, x, y)); }
+
+// Godunov flux with kp = rho c^2 / 2, kc = c / 2, kr = 1/(2 rho):
+// Fp = kp (uR + uL) - kc (pR - pL), Fu = kr (pR + pL) - kc (uR - uL).
+template <typename R> struct TileK {
+    Split<R> kp, kr, dth, third;   // rho c^2/2, 1/(2 rho), dt/h, 1/3
+    R kc, ih;                       // c/2 (exact), 1/h (PML only)
+};
+template <typename R>
+__device__ inline void godunov_f(R pL, R pR, R uL, R uR, const TileK<R> &K, R &Fp, R &Fu) {
+    Fp = fma_s(K.kp, uR + uL, -K.kc * (pR - pL));
+    Fu = fma_s(K.kr, pR + pL, -K.kc * (uR - uL));
 }
 
 template <typename R>
@@ -89,15 +109,17 @@ __device__ inline R shfl_next(R x) { return __shfl_down(x, 1, 64); }   // value 
 template <typename R>
 __device__ inline R shfl_prev(R x) { return __shfl_up(x, 1, 64); }     // value of lane - 1
 
-template <typename S, typename R, int STAGE>
-__global__ void __launch_bounds__(64 * tile::NW)
+// One block = 64 x NW threads; outputs 62 columns x (NW-2) rows x TZ planes.
+template <typename S, typename R, int STAGE, int NW>
+__global__ void __launch_bounds__(64 * NW)
 k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restrict__ iu, const S *__restrict__ iv,
              const S *__restrict__ iw, const S *__restrict__ bp, const S *__restrict__ bu, const S *__restrict__ bv,
              const S *__restrict__ bw, S *__restrict__ op, S *__restrict__ ou, S *__restrict__ ov,
-             S *__restrict__ ow, const uint8_t *__restrict__ upd, PmlSet pml, R rho, R c, R h, R dt, R s_in,
+             S *__restrict__ ow, const uint8_t *__restrict__ upd, PmlSet pml, TileK<R> K, R dt, R s_in,
              R s_base, R is_out) {
     using namespace tile;
-    __shared__ TileLds<R> L;
+    constexpr int LY = NW + 4, TYO = NW - 2;
+    __shared__ TileLds<R, NW> L;
     typedef Conv<S, R> CV;
     const int lx = threadIdx.x, wy = threadIdx.y;
     const int tid = wy * 64 + lx;
@@ -106,13 +128,13 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
     const int kend = min(k0 + TZ, Nz - 3);
     const int i = x0 - 1 + lx, j = j0 - 1 + wy;
     const int ic = min(i, Nx - 1), jc = min(j, Ny - 1);               // clamped for loads
-    const long long col = ic + (long long)Nx * jc;
+    const unsigned col = (unsigned)(ic + Nx * jc);                   // offset within a plane
     const bool inner_row = (wy >= 1 && wy <= NW - 2);
     const bool out_lane = (lx >= 1 && lx <= TXO) && i < Nx - 3 && j < Ny - 3 && inner_row;
 
-    // halo item of this thread: p columns/rows, u columns, v rows (see LDS layout)
+    // halo item of this thread: p columns/rows, u columns, v rows
     int h_kind = -1, h_r = 0, h_c = 0;
-    long long h_off = 0;
+    unsigned h_off = 0;
     {
         const int nP1 = 4 * LY, nP2 = 4 * 64, nU = 4 * NW, nV = 4 * 64;
         const int sideX[4] = {0, 1, LX - 2, LX - 1};
@@ -128,15 +150,15 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
             h_kind = 2; h_r = sideY[t / 64]; h_c = t % 64;
         }
         if (h_kind >= 0) {
-            // grid coordinates of the halo cell (columns x0-3+c for p/u, x0-1+c for v;
-            // rows j0-3+r for p/v, j0-1+r for u), clamped into the grid
             int gx = (h_kind == 2) ? (x0 - 1 + h_c) : (x0 - 3 + h_c);
             int gy = (h_kind == 1) ? (j0 - 1 + h_r) : (j0 - 3 + h_r);
             gx = max(0, min(gx, Nx - 1));
             gy = max(0, min(gy, Ny - 1));
-            h_off = gx + (long long)Nx * gy;
+            h_off = (unsigned)(gx + Nx * gy);
         }
     }
+    const S *hsrc = (h_kind == 0) ? ip : (h_kind == 1 ? iu : iv);
+    R *hdst = (h_kind == 0) ? &L.p[h_r][h_c] : (h_kind == 1 ? &L.u[h_r][h_c] : &L.v[h_r][h_c]);
 
     // z pipeline: window p/w at k-1..k+3 (index 0..4), carried A-state of cell k and
     // flux through k-1/2
@@ -146,51 +168,65 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
         R p7[7], w7[7];
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
-            const long long off = col + nxy * (long long)(k0 - 3 + q);
-            p7[q] = CV::ld(ip[off], s_in);
-            w7[q] = CV::ld(iw[off], s_in);
+            const long long pq = nxy * (long long)(k0 - 3 + q);
+            p7[q] = CV::ld((ip + pq)[col], s_in);
+            w7[q] = CV::ld((iw + pq)[col], s_in);
         }
         R pAm, wAm, pB0, wB0, tmp0, tmp1;
         weno_pair(p7[0], p7[1], p7[2], p7[3], p7[4], pAm, tmp0);   // cell k0-1
         weno_pair(w7[0], w7[1], w7[2], w7[3], w7[4], wAm, tmp1);
         weno_pair(p7[1], p7[2], p7[3], p7[4], p7[5], pA, pB0);     // cell k0
         weno_pair(w7[1], w7[2], w7[3], w7[4], w7[5], wA, wB0);
-        godunov_f(pAm, pB0, wAm, wB0, rho, c, FzP, FzW);              // face k0-1/2
+        godunov_f(pAm, pB0, wAm, wB0, K, FzP, FzW);            // face k0-1/2
 #pragma unroll
         for (int q = 0; q < 5; ++q) { pz[q] = p7[q + 2]; wz[q] = w7[q + 2]; }
+    }
+    // plane values in flight: own u, v (and p for the two outer rows), halo item
+    R nu, nv, np = 0, nh = 0;
+    {
+        const long long pl = nxy * (long long)k0;
+        nu = CV::ld((iu + pl)[col], s_in);
+        nv = CV::ld((iv + pl)[col], s_in);
+        if (!inner_row) np = CV::ld((ip + pl)[col], s_in);
+        if (h_kind >= 0) nh = CV::ld((hsrc + pl)[h_off], s_in);
     }
 
     for (int k = k0; k < kend; ++k) {
         const long long pl = nxy * (long long)k;
-        // ---- fill the plane tiles
-        const R pc = inner_row ? pz[1] : CV::ld(ip[col + pl], s_in);
+        // ---- fill the plane tiles from the values loaded during the previous plane
+        const R pc = inner_row ? pz[1] : np;
+        const R uc = nu, vc = nv;
         L.p[wy + 2][lx + 2] = pc;
-        L.u[wy][lx + 2] = CV::ld(iu[col + pl], s_in);
-        L.v[wy + 2][lx] = CV::ld(iv[col + pl], s_in);
-        if (h_kind == 0) L.p[h_r][h_c] = CV::ld(ip[h_off + pl], s_in);
-        else if (h_kind == 1) L.u[h_r][h_c] = CV::ld(iu[h_off + pl], s_in);
-        else if (h_kind == 2) L.v[h_r][h_c] = CV::ld(iv[h_off + pl], s_in);
+        L.u[wy][lx + 2] = uc;
+        L.v[wy + 2][lx] = vc;
+        if (h_kind >= 0) *hdst = nh;
         __syncthreads();
-
+        // ---- prefetch the next plane
+        if (k + 1 < kend) {
+            const long long pn = pl + nxy;
+            nu = CV::ld((iu + pn)[col], s_in);
+            nv = CV::ld((iv + pn)[col], s_in);
+            if (!inner_row) np = CV::ld((ip + pn)[col], s_in);
+            if (h_kind >= 0) nh = CV::ld((hsrc + pn)[h_off], s_in);
+        }
         // ---- y states of every row (incl. the two outer rows)
         {
             R A, B, Av, Bv;
             weno_pair(L.p[wy][lx + 2], L.p[wy + 1][lx + 2], pc, L.p[wy + 3][lx + 2], L.p[wy + 4][lx + 2], A, B);
-            weno_pair(L.v[wy][lx], L.v[wy + 1][lx], L.v[wy + 2][lx], L.v[wy + 3][lx], L.v[wy + 4][lx], Av, Bv);
+            weno_pair(L.v[wy][lx], L.v[wy + 1][lx], vc, L.v[wy + 3][lx], L.v[wy + 4][lx], Av, Bv);
             L.ys[0][wy][lx] = A;
             L.ys[1][wy][lx] = B;
             L.ys[2][wy][lx] = Av;
             L.ys[3][wy][lx] = Bv;
         }
         // ---- x faces (shuffles are wave-wide: every lane of an inner row takes part)
-        R FxP_p = 0, FxP_u = 0, FxM_p = 0, FxM_u = 0, uc = 0;
+        R FxP_p = 0, FxP_u = 0, FxM_p = 0, FxM_u = 0;
         if (inner_row) {
-            uc = L.u[wy][lx + 2];
             R A, B, Au, Bu;
             weno_pair(L.p[wy + 2][lx], L.p[wy + 2][lx + 1], pc, L.p[wy + 2][lx + 3], L.p[wy + 2][lx + 4], A, B);
             weno_pair(L.u[wy][lx], L.u[wy][lx + 1], uc, L.u[wy][lx + 3], L.u[wy][lx + 4], Au, Bu);
             const R Bn = shfl_next(B), Bun = shfl_next(Bu);
-            godunov_f(A, Bn, Au, Bun, rho, c, FxP_p, FxP_u);          // face i+1/2
+            godunov_f(A, Bn, Au, Bun, K, FxP_p, FxP_u);          // face i+1/2
             FxM_p = shfl_prev(FxP_p);
             FxM_u = shfl_prev(FxP_u);
         }
@@ -199,61 +235,63 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
         if (inner_row) {
             // ---- y faces j+1/2 and j-1/2
             R FyP_p, FyP_v, FyM_p, FyM_v;
-            godunov_f(L.ys[0][wy][lx], L.ys[1][wy + 1][lx], L.ys[2][wy][lx], L.ys[3][wy + 1][lx], rho, c, FyP_p, FyP_v);
-            godunov_f(L.ys[0][wy - 1][lx], L.ys[1][wy][lx], L.ys[2][wy - 1][lx], L.ys[3][wy][lx], rho, c, FyM_p, FyM_v);
-            const R vc = L.v[wy + 2][lx];
+            godunov_f(L.ys[0][wy][lx], L.ys[1][wy + 1][lx], L.ys[2][wy][lx], L.ys[3][wy + 1][lx], K, FyP_p, FyP_v);
+            godunov_f(L.ys[0][wy - 1][lx], L.ys[1][wy][lx], L.ys[2][wy - 1][lx], L.ys[3][wy][lx], K, FyM_p, FyM_v);
             // ---- z face k+1/2 from the new cell k+1
             R pAn, pBn, wAn, wBn;
             weno_pair(pz[0], pz[1], pz[2], pz[3], pz[4], pAn, pBn);
             weno_pair(wz[0], wz[1], wz[2], wz[3], wz[4], wAn, wBn);
             R FzP_p, FzP_w;
-            godunov_f(pA, pBn, wA, wBn, rho, c, FzP_p, FzP_w);
+            godunov_f(pA, pBn, wA, wBn, K, FzP_p, FzP_w);
             const R wc = wz[1];
-            const long long C = col + pl;
-            if (out_lane && upd[C] == 1) {
+            // z window for the next plane: issue the lead loads before the update
+            const R pl4 = pz[4], wl4 = wz[4];
+            R pnew = 0, wnew = 0;
+            if (k + 1 < kend) {
+                const long long p4 = nxy * (long long)(k + 4);
+                pnew = CV::ld((ip + p4)[col], s_in);
+                wnew = CV::ld((iw + p4)[col], s_in);
+            }
+            if (out_lane && (upd + pl)[col] == 1) {
                 const R sp = -FxP_p + FxM_p - FyP_p + FyM_p - FzP_p + FzP;
+                const R Du = -FxP_u + FxM_u, Dv = -FyP_v + FyM_v, Dw = -FzP_w + FzW;
                 R uo, vo, wo, po, wgt;
                 if (STAGE == 0) {
-                    uo = uc + dt * (-FxP_u + FxM_u) / h;
-                    vo = vc + dt * (-FyP_v + FyM_v) / h;
-                    wo = wc + dt * (-FzP_w + FzW) / h;
-                    po = pc + dt * sp / h;
+                    uo = fma_s(K.dth, Du, uc);
+                    vo = fma_s(K.dth, Dv, vc);
+                    wo = fma_s(K.dth, Dw, wc);
+                    po = fma_s(K.dth, sp, pc);
                     wgt = R(1);
                 } else {
-                    const R p0 = CV::ld(bp[C], s_base), u0 = CV::ld(bu[C], s_base);
-                    const R v0 = CV::ld(bv[C], s_base), w0 = CV::ld(bw[C], s_base);
+                    const R p0 = CV::ld((bp + pl)[col], s_base), u0 = CV::ld((bu + pl)[col], s_base);
+                    const R v0 = CV::ld((bv + pl)[col], s_base), w0 = CV::ld((bw + pl)[col], s_base);
                     if (STAGE == 1) {
-                        uo = R(0.75) * u0 + R(0.25) * (uc + dt * ((-FxP_u + FxM_u) / h));
-                        vo = R(0.75) * v0 + R(0.25) * (vc + dt * ((-FyP_v + FyM_v) / h));
-                        wo = R(0.75) * w0 + R(0.25) * (wc + dt * ((-FzP_w + FzW) / h));
-                        po = R(0.75) * p0 + R(0.25) * (pc + dt * (sp / h));
+                        uo = fma(R(0.75), u0, R(0.25) * fma_s(K.dth, Du, uc));
+                        vo = fma(R(0.75), v0, R(0.25) * fma_s(K.dth, Dv, vc));
+                        wo = fma(R(0.75), w0, R(0.25) * fma_s(K.dth, Dw, wc));
+                        po = fma(R(0.75), p0, R(0.25) * fma_s(K.dth, sp, pc));
                         wgt = R(0.25);
                     } else {
-                        uo = (u0 + R(2) * (uc + dt * ((-FxP_u + FxM_u) / h))) / R(3);
-                        vo = (v0 + R(2) * (vc + dt * ((-FyP_v + FyM_v) / h))) / R(3);
-                        wo = (w0 + R(2) * (wc + dt * ((-FzP_w + FzW) / h))) / R(3);
-                        po = (p0 + R(2) * (pc + dt * (sp / h))) / R(3);
+                        uo = mul_s(K.third, fma(R(2), fma_s(K.dth, Du, uc), u0));
+                        vo = mul_s(K.third, fma(R(2), fma_s(K.dth, Dv, vc), v0));
+                        wo = mul_s(K.third, fma(R(2), fma_s(K.dth, Dw, wc), w0));
+                        po = mul_s(K.third, fma(R(2), fma_s(K.dth, sp, pc), p0));
                         wgt = R(2) / R(3);
                     }
                 }
                 if (pml.on)
-                    pml_apply<R, STAGE == 2>(pml, i, j, k, Nx, Ny, po, uo, vo, wo, (-FxP_p + FxM_p) / h,
-                                             (-FyP_p + FyM_p) / h, (-FzP_p + FzP) / h, (-FxP_u + FxM_u) / h,
-                                             (-FyP_v + FyM_v) / h, (-FzP_w + FzW) / h, wgt * dt);
-                ou[C] = CV::st(uo, is_out);
-                ov[C] = CV::st(vo, is_out);
-                ow[C] = CV::st(wo, is_out);
-                op[C] = CV::st(po, is_out);
+                    pml_apply<R, STAGE == 2>(pml, i, j, k, Nx, Ny, po, uo, vo, wo, (-FxP_p + FxM_p) * K.ih,
+                                             (-FyP_p + FyM_p) * K.ih, (-FzP_p + FzP) * K.ih, Du * K.ih, Dv * K.ih,
+                                             Dw * K.ih, wgt * dt);
+                (ou + pl)[col] = CV::st(uo, is_out);
+                (ov + pl)[col] = CV::st(vo, is_out);
+                (ow + pl)[col] = CV::st(wo, is_out);
+                (op + pl)[col] = CV::st(po, is_out);
             }
             // advance the z pipeline
             pA = pAn; wA = wAn; FzP = FzP_p; FzW = FzP_w;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) { pz[q] = pz[q + 1]; wz[q] = wz[q + 1]; }
-            if (k + 1 < kend) {
-                const long long off = col + nxy * (long long)(k + 4);
-                pz[4] = CV::ld(ip[off], s_in);
-                wz[4] = CV::ld(iw[off], s_in);
-            }
+            pz[0] = pz[1]; pz[1] = pz[2]; pz[2] = pz[3]; pz[3] = pl4; pz[4] = pnew;
+            wz[0] = wz[1]; wz[1] = wz[2]; wz[2] = wz[3]; wz[3] = wl4; wz[4] = wnew;
         }
     }
 }
