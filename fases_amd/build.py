@@ -58,6 +58,13 @@ def build_core() -> Path:
     return out
 
 
+def build_cli() -> Path:
+    out = PKG / "fases3D"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", str(CSRC / "fases3d_main.cpp"), "-o", str(out),
+          "-L" + str(PKG), "-lfases_core", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def build_oracle() -> Path:
     d = ROOT / "oracle" / "_build"
     d.mkdir(exist_ok=True)
@@ -73,6 +80,7 @@ def build_reference_checker() -> None:
 
 def build_all() -> None:
     build_core()
+    build_cli()
     build_oracle()
     build_reference_checker()
 
