@@ -43,6 +43,7 @@ class fases_params(C.Structure):
         ("fp16_rescale_every", C.c_int),
         ("use_graph", C.c_int),
         ("kernel_variant", C.c_int),
+        ("tile_z", C.c_int),
     ]
 
 
@@ -91,6 +92,8 @@ SIGNATURES = {
     "fases_reset": (C.c_int, [C.c_void_p]),
     "fases_run": (C.c_int, [C.c_void_p, C.c_int64, _dbl, _dbl, _dbl, _dbl]),
     "fases_info": (C.c_int, [C.c_void_p, _i64, _i64, _i64, _i64]),
+    "fases_tile_config": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
+    "fases_batch_run": (C.c_int, [_P(C.c_void_p), C.c_int, _i64, _P(_P(C.c_double)), _P(C.c_double)]),
 }
 
 _lib = None

@@ -1,0 +1,252 @@
+// Solver context (the C ABI's opaque fases_ctx) and the host helpers shared by the
+// single-context and batched time loops.
+#pragma once
+
+#include "fvm_kernels.hpp"
+#include "host.hpp"
+#include "../../include/fases_core.h"
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+using namespace fasesdev;
+
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);             \
+            return FASES_E_HIP;                                                       \
+        }                                                                             \
+    } while (0)
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+
+struct fases_ctx {
+    fases_params prm;
+    std::string err;
+    hipStream_t stream = nullptr;
+    bool have_geo = false, have_mat = false;
+    int Nx = 0, Ny = 0, Nz = 0;
+    long long N = 0, n_upd = 0;
+    size_t esz = 8;                      // bytes per stored field value
+    std::vector<DevBuf> bufs;            // every device allocation, freed at destroy
+    size_t dev_bytes = 0;
+    void *U[3][4] = {};                  // three field sets x (p, u, v, w)
+    uint8_t *d_upd = nullptr;
+    std::vector<uint8_t> h_upd;
+    int n_patch = 0;                     // cells kept in FP64 shadows (near sources)
+    int *d_patch = nullptr;              // their grid indices
+    int *d_pidx = nullptr;               // grid -> patch index, -1 elsewhere (allocated on demand)
+    double *shadow[3] = {};              // FP64 shadow per field set, [4][n_patch]
+    std::vector<int> h_patch;
+    // boundary list
+    int nB = 0;
+    int *d_bcell = nullptr, *d_bbase = nullptr, *d_bmat = nullptr;
+    double *d_bgeo = nullptr;
+    std::vector<int> h_bcell, h_bmat;
+    // materials and ADE states
+    int nMat = 0, K = 1, L = 1;
+    double *d_Rinf = nullptr, *d_A = nullptr, *d_zeta = nullptr, *d_B = nullptr, *d_C = nullptr,
+           *d_alpha = nullptr, *d_beta = nullptr;
+    int *d_Keff = nullptr, *d_Leff = nullptr;
+    double *phi[3] = {}, *psi1[3] = {}, *psi2[3] = {};   // [0]=phi, [1]=phi_old, [2]=phi_nu
+    // PML
+    PmlSet pml{};
+    // sources / receivers
+    int n_src = 0, src_opt = FASES_SRC_RICKER;
+    long long n_samp = 0;
+    int *d_scell = nullptr;
+    double *d_sval = nullptr, *d_sfreq = nullptr, *d_us = nullptr;
+    std::vector<double> h_sfreq;         // sine sources: frequencies (samples tabulated per run)
+    int n_rec = 0;
+    int *d_rcell = nullptr;
+    double *d_ir = nullptr, *d_vel = nullptr;
+    long long ir_cap = 0;
+    long long step = 0;
+    std::vector<uint8_t> h_oid;
+    // FP16 storage: field scale exponents (value = stored * 2^fexp[f]) and the reduction
+    // buffer of the rescale pass (4 maxima + non-finite flag)
+    int fexp[4] = {0, 0, 0, 0};
+    unsigned *d_amax = nullptr;
+    int tz = 0;                          // z planes per block of the tiled kernel (0: not tuned)
+
+    ~fases_ctx() {
+        for (auto &b : bufs) if (b.ptr) (void)hipFree(b.ptr);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+int dalloc(fases_ctx *ctx, T **out, size_t count) {
+    *out = nullptr;
+    if (count == 0) count = 1;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipMalloc(") + std::to_string(count * sizeof(T)) + " B): " + hipGetErrorString(e);
+        return FASES_E_NOMEM;
+    }
+    e = hipMemsetAsync(p, 0, count * sizeof(T), ctx->stream);
+    if (e != hipSuccess) { (void)hipFree(p); ctx->err = hipGetErrorString(e); return FASES_E_HIP; }
+    ctx->bufs.push_back({p, count * sizeof(T)});
+    ctx->dev_bytes += count * sizeof(T);
+    *out = static_cast<T *>(p);
+    return FASES_OK;
+}
+
+void dfree_all(fases_ctx *ctx, void *keep_none = nullptr) {
+    (void)keep_none;
+    for (auto &b : ctx->bufs) if (b.ptr) (void)hipFree(b.ptr);
+    ctx->bufs.clear();
+    ctx->dev_bytes = 0;
+}
+
+template <typename T>
+int upload(fases_ctx *ctx, T **dst, const T *src, size_t count) {
+    int rc = dalloc(ctx, dst, count);
+    if (rc) return rc;
+    if (count) HIPCHK(hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return FASES_OK;
+}
+
+int set_error(fases_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    else fases::set_global_error(msg);
+    return code;
+}
+
+template <typename S>
+FieldSet<S> fset(fases_ctx *ctx, int s) {
+    return FieldSet<S>{(S *)ctx->U[s][0], (S *)ctx->U[s][1], (S *)ctx->U[s][2], (S *)ctx->U[s][3]};
+}
+
+template <typename R>
+Scl<R> field_scales(const fases_ctx *ctx) {
+    Scl<R> sc;
+    for (int f = 0; f < 4; ++f) {
+        sc.s[f] = (R)std::ldexp(1.0, ctx->fexp[f]);
+        sc.is[f] = (R)std::ldexp(1.0, -ctx->fexp[f]);
+    }
+    return sc;
+}
+
+template <typename R>
+Split<R> split_const(double v) {
+    const R hi = (R)v;
+    return Split<R>{hi, (R)(v - (double)hi)};
+}
+
+template <typename R>
+TileK<R> tile_constants(const fases_params &P) {
+    TileK<R> K;
+    K.kp = split_const<R>(0.5 * P.rho * P.c * P.c);
+    K.kr = split_const<R>(0.5 / P.rho);
+    K.dth = split_const<R>(P.dt / P.h);
+    K.third = split_const<R>(1.0 / 3.0);
+    K.kc = (R)(0.5 * P.c);
+    K.ih = (R)(1.0 / P.h);
+    return K;
+}
+
+template <typename S, typename R, int NW>
+int launch_tiled(fases_ctx *ctx, int stage, int in, int out, R rho, R c, R h, R dt, int tz) {
+    (void)rho; (void)c; (void)h;
+    const TileK<R> K = tile_constants<R>(ctx->prm);
+    const FieldSet<S> fi = fset<S>(ctx, in), fb = fset<S>(ctx, 0), fo = fset<S>(ctx, out);
+    const dim3 tb(64, NW);
+    const dim3 tg((ctx->Nx - 6 + tile::TXO - 1) / tile::TXO, (ctx->Ny - 6 + (NW - 2) - 1) / (NW - 2),
+                  (ctx->Nz - 6 + tz - 1) / tz);
+    const Scl<R> sc = field_scales<R>(ctx);
+    const TileFields<S> F{fi.p, fi.u, fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w};
+    if (stage == 0)
+        hipLaunchKernelGGL((k_flux_tiled<S, R, 0, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
+                           ctx->d_upd, ctx->pml, K, dt, sc, tz);
+    else if (stage == 1)
+        hipLaunchKernelGGL((k_flux_tiled<S, R, 1, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
+                           ctx->d_upd, ctx->pml, K, dt, sc, tz);
+    else
+        hipLaunchKernelGGL((k_flux_tiled<S, R, 2, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
+                           ctx->d_upd, ctx->pml, K, dt, sc, tz);
+    return FASES_OK;
+}
+
+bool uses_tiled(const fases_ctx *ctx) {
+    const int kv = ctx->prm.kernel_variant;
+    return !(kv == 1 || (kv == 0 && ctx->prm.precision == FASES_FP64));
+}
+
+// Tile autotuner: times the stage-0 flux launch (set 0 -> set 1; set 1 is rewritten by
+// the next step's stage 0, and stage 0 leaves the PML states alone, so this changes no
+// state) for each candidate z depth, three times each, and keeps the fastest.
+template <typename S, typename R>
+int autotune_tile(fases_ctx *ctx) {
+    if (ctx->prm.tile_z > 0) { ctx->tz = ctx->prm.tile_z; return FASES_OK; }
+    const fases_params &P = ctx->prm;
+    const int nz = ctx->Nz - 6;
+    const int cand[4] = {16, 32, 64, 128};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float best = 1e30f;
+    int pick = 64;
+    for (int tz : cand) {
+        if (tz > 16 && (tz / 2) >= nz) break;   // deeper blocks than the grid add nothing
+        float tmin = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIPCHK(hipEventRecord(e0, ctx->stream));
+            int rc = launch_tiled<S, R, 16>(ctx, 0, 0, 1, (R)P.rho, (R)P.c, (R)P.h, (R)P.dt, tz);
+            if (rc) return rc;
+            HIPCHK(hipEventRecord(e1, ctx->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            tmin = std::min(tmin, ms);
+        }
+        if (tmin < best) { best = tmin; pick = tz; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipGetLastError());
+    ctx->tz = pick;
+    return FASES_OK;
+}
+
+// Per-run set-up shared by the single-context and batched loops: sine sample tables
+// long enough for the run, a waveform-length check, receiver buffers of nsteps rows.
+inline int prepare_run(fases_ctx *ctx, long long nsteps) {
+    if (ctx->n_src > 0 && ctx->src_opt == FASES_SRC_SINE && 2 * (ctx->step + nsteps) + 1 > ctx->n_samp) {
+        // per-source sine samples sin(2 pi f nn dt/2), tabulated on the host (same libm
+        // as a CPU evaluation of the reference formula)
+        const long long ns = 2 * (ctx->step + nsteps) + 1;
+        std::vector<double> tab((size_t)ctx->n_src * ns);
+        for (int q = 0; q < ctx->n_src; ++q)
+            for (long long t = 0; t < ns; ++t)
+                tab[(size_t)q * ns + t] = std::sin(2.0 * M_PI * ctx->h_sfreq[(size_t)q] * (double)t * (0.5 * ctx->prm.dt));
+        int rc = upload(ctx, &ctx->d_us, tab.data(), tab.size());
+        if (rc) return rc;
+        ctx->n_samp = ns;
+    }
+    if (ctx->n_rec > 0 && nsteps > ctx->ir_cap) {
+        // grow the device receiver buffers
+        int rc = dalloc(ctx, &ctx->d_ir, (size_t)ctx->n_rec * nsteps);
+        if (rc) return rc;
+        rc = dalloc(ctx, &ctx->d_vel, (size_t)ctx->n_rec * nsteps * 3);
+        if (rc) return rc;
+        ctx->ir_cap = nsteps;
+    }
+    if (ctx->n_src > 0 && ctx->src_opt != FASES_SRC_SINE && 2 * (ctx->step + nsteps) + 1 > ctx->n_samp)
+        return set_error(ctx, FASES_E_ARG, "source waveform shorter than 2*(steps)+1 samples");
+    return FASES_OK;
+}
+
+}  // namespace

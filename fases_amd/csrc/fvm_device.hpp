@@ -36,6 +36,12 @@ template <> struct Conv<__half, float> {
     __device__ static inline __half st(float x, float inv_scale) { return __float2half_rn(x * inv_scale); }
 };
 
+// Storage scales of the four fields (p, vx, vy, vz), shared by the three field sets:
+// value = stored * s[f], stored = value * is[f]. Powers of two; 1 for FP32 / FP64.
+template <typename R> struct Scl {
+    R s[4], is[4];
+};
+
 // Literal WENO5-Z of the reference for the two states of face i+1/2, from the six
 // cells u[i-2..i+3] = (a,b,c,d,e,f):
 //   left  = V^-_{i+1/2}, reconstruction of cell i   (idx = 1 in functions3D.cuh)

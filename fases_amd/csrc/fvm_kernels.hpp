@@ -1,0 +1,395 @@
+// Device kernels and parameter blocks of the FVM step shared by the single-context
+// time loop (fvm_context.hip) and the batched multi-job loop (fvm_batch.hip).
+#pragma once
+
+#include "fvm_device.hpp"
+#include "fvm_tiled.hpp"
+
+#include <hip/hip_runtime.h>
+#include <type_traits>
+
+namespace fasesdev {
+
+constexpr int NG = 3;   // ghost cells per side (parsing3D.h:128)
+
+// ------------------------------------------------------------------------------------
+// Kernel parameter blocks
+// ------------------------------------------------------------------------------------
+
+template <typename S> struct FieldSet {
+    S *p, *u, *v, *w;
+};
+
+struct Grid {
+    int Nx, Ny, Nz;
+    long long nxy;
+};
+
+// ------------------------------------------------------------------------------------
+// Flux stage: WENO5-Z + Godunov + SSP-RK3 stage combination + cPML correction.
+// STAGE 0: out = U0 + dt L(U0)                       (functions3D.cuh:1149-1152)
+// STAGE 1: out = 3/4 U0 + 1/4 (in + dt L(in))        (:1230-1233)
+// STAGE 2: out = (U0 + 2 (in + dt L(in))) / 3        (:1311-1314)
+// One thread per cell; the cell is updated when i,j,k in [3, N-3) and S >= h.
+// ------------------------------------------------------------------------------------
+template <typename S>
+__device__ inline S *fld(const FieldSet<S> &s, int f) {
+    return f == 0 ? s.p : (f == 1 ? s.u : (f == 2 ? s.v : s.w));
+}
+
+// Field access of the grid kernel: stored fields in, stored fields out.
+template <typename S, typename R> struct GridIO {
+    FieldSet<S> in, base, out;
+    Scl<R> sc;
+    __device__ R ld_in(int f, long long C) const { return Conv<S, R>::ld(fld(in, f)[C], sc.s[f]); }
+    __device__ R ld_base(int f, long long C) const { return Conv<S, R>::ld(fld(base, f)[C], sc.s[f]); }
+    __device__ void st(int f, long long C, R v) const { fld(out, f)[C] = Conv<S, R>::st(v, sc.is[f]); }
+};
+
+// Field access of the source-patch kernel: cells with a patch index read and write their
+// FP64 shadow (and refresh the stored copy); other cells read the stored fields.
+template <typename S> struct PatchIO {
+    FieldSet<S> in, base, out;
+    const int *pidx;
+    const double *sh_in, *sh_base;
+    double *sh_out;
+    int np;
+    Scl<double> sc;
+    __device__ double ld_in(int f, long long C) const {
+        const int q = pidx[C];
+        return q >= 0 ? sh_in[(long long)f * np + q] : Conv<S, double>::ld(fld(in, f)[C], sc.s[f]);
+    }
+    __device__ double ld_base(int f, long long C) const {
+        const int q = pidx[C];
+        return q >= 0 ? sh_base[(long long)f * np + q] : Conv<S, double>::ld(fld(base, f)[C], sc.s[f]);
+    }
+    __device__ void st(int f, long long C, double v) const {
+        sh_out[(long long)f * np + pidx[C]] = v;
+        fld(out, f)[C] = Conv<S, double>::st(v, sc.is[f]);
+    }
+};
+
+template <typename IO, typename R, int STAGE>
+__device__ inline void flux_cell(const Grid &g, int i, int j, int k, const IO &io, const PmlSet &pml, R rho, R c,
+                                 R h, R dt) {
+#pragma clang fp contract(off)
+    const long long C = (long long)i + (long long)g.Nx * j + g.nxy * k;
+
+    // fluxes through the faces C+1/2 (P) and C-1/2 (M) along one axis
+    auto flux_axis = [&](int fu, long long st, R &Fp_p, R &Fp_m, R &Fu_p, R &Fu_m) {
+        R pw[7], uw[7];
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            pw[q] = io.ld_in(0, C + (q - 3) * st);
+            uw[q] = io.ld_in(fu, C + (q - 3) * st);
+        }
+        R pl, pr, ul, ur;
+        weno_face(pw[1], pw[2], pw[3], pw[4], pw[5], pw[6], pl, pr);   // face +1/2
+        weno_face(uw[1], uw[2], uw[3], uw[4], uw[5], uw[6], ul, ur);
+        godunov(pl, pr, ul, ur, rho, c, Fp_p, Fu_p);
+        weno_face(pw[0], pw[1], pw[2], pw[3], pw[4], pw[5], pl, pr);   // face -1/2
+        weno_face(uw[0], uw[1], uw[2], uw[3], uw[4], uw[5], ul, ur);
+        godunov(pl, pr, ul, ur, rho, c, Fp_m, Fu_m);
+    };
+
+    R FpP, FpM, FuP, FuM, GpP, GpM, GvP, GvM, HpP, HpM, HwP, HwM;
+    flux_axis(1, 1, FpP, FpM, FuP, FuM);
+    flux_axis(2, g.Nx, GpP, GpM, GvP, GvM);
+    flux_axis(3, g.nxy, HpP, HpM, HwP, HwM);
+
+    const R p_in = io.ld_in(0, C), u_in = io.ld_in(1, C), v_in = io.ld_in(2, C), w_in = io.ld_in(3, C);
+    const R sp = -FpP + FpM - GpP + GpM - HpP + HpM;
+    R uo, vo, wo, po, wgt;
+    if (STAGE == 0) {
+        uo = u_in + dt * (-FuP + FuM) / h;
+        vo = v_in + dt * (-GvP + GvM) / h;
+        wo = w_in + dt * (-HwP + HwM) / h;
+        po = p_in + dt * sp / h;
+        wgt = R(1);
+    } else {
+        const R p0 = io.ld_base(0, C), u0 = io.ld_base(1, C), v0 = io.ld_base(2, C), w0 = io.ld_base(3, C);
+        if (STAGE == 1) {
+            uo = R(0.75) * u0 + R(0.25) * (u_in + dt * ((-FuP + FuM) / h));
+            vo = R(0.75) * v0 + R(0.25) * (v_in + dt * ((-GvP + GvM) / h));
+            wo = R(0.75) * w0 + R(0.25) * (w_in + dt * ((-HwP + HwM) / h));
+            po = R(0.75) * p0 + R(0.25) * (p_in + dt * (sp / h));
+            wgt = R(0.25);
+        } else {
+            uo = (u0 + R(2) * (u_in + dt * ((-FuP + FuM) / h))) / R(3);
+            vo = (v0 + R(2) * (v_in + dt * ((-GvP + GvM) / h))) / R(3);
+            wo = (w0 + R(2) * (w_in + dt * ((-HwP + HwM) / h))) / R(3);
+            po = (p0 + R(2) * (p_in + dt * (sp / h))) / R(3);
+            wgt = R(2) / R(3);
+        }
+    }
+    if (pml.on)
+        pml_apply<R, STAGE == 2>(pml, i, j, k, g.Nx, g.Ny, po, uo, vo, wo, (-FpP + FpM) / h, (-GpP + GpM) / h,
+                                 (-HpP + HpM) / h, (-FuP + FuM) / h, (-GvP + GvM) / h, (-HwP + HwM) / h, wgt * dt);
+    io.st(1, C, uo);
+    io.st(2, C, vo);
+    io.st(3, C, wo);
+    io.st(0, C, po);
+}
+
+// Update-mask values: 0 = not updated (outside [3,N-3)^3 or S < h), 1 = updated by the
+// grid kernel, 2 = updated by the patch kernel (reduced-precision storage only: cells
+// near point sources, kept in FP64 shadows, see fases_set_sources).
+template <typename S, typename R, int STAGE>
+__global__ void __launch_bounds__(256)
+k_flux_simple(Grid g, GridIO<S, R> io, const uint8_t *__restrict__ upd, PmlSet pml, R rho, R c, R h, R dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + NG;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + NG;
+    const int k = blockIdx.z + NG;
+    if (i >= g.Nx - NG || j >= g.Ny - NG || k >= g.Nz - NG) return;
+    const long long C = (long long)i + (long long)g.Nx * j + g.nxy * k;
+    if (upd[C] != 1) return;
+    flux_cell<GridIO<S, R>, R, STAGE>(g, i, j, k, io, pml, rho, c, h, dt);
+}
+
+// The same stage update in FP64 arithmetic and FP64 shadow storage on the patch cells.
+template <typename S, int STAGE>
+__global__ void __launch_bounds__(64)
+k_flux_patch(Grid g, PatchIO<S> io, const int *__restrict__ cells, PmlSet pml, double rho, double c, double h,
+             double dt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= io.np) return;
+    const int C = cells[t];
+    const int i = C % g.Nx, j = (C / g.Nx) % g.Ny, k = (int)(C / g.nxy);
+    flux_cell<PatchIO<S>, double, STAGE>(g, i, j, k, io, pml, rho, c, h, dt);
+}
+
+// Copies the stored fields of the patch cells into an FP64 shadow set.
+template <typename S>
+__global__ void k_patch_gather(FieldSet<S> f, const int *__restrict__ cells, int np, double *__restrict__ sh,
+                               Scl<double> sc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    const int C = cells[t];
+    for (int q = 0; q < 4; ++q) sh[(long long)q * np + t] = Conv<S, double>::ld(fld(f, q)[C], sc.s[q]);
+}
+
+// Largest |stored value| of each field of one set and a non-finite flag:
+// out[0..3] = float bit patterns (non-negative floats order like unsigned ints), out[4] != 0
+// when a value is Inf or NaN. out[] must be zero on entry.
+template <typename S>
+__global__ void __launch_bounds__(256) k_absmax(FieldSet<S> f, long long n, unsigned *__restrict__ out) {
+    float m[4] = {0.f, 0.f, 0.f, 0.f};
+    unsigned bad = 0;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float x = fabsf((float)fld(f, q)[c]);
+            if (!(x <= 3.0e38f)) bad = 1;
+            else m[q] = fmaxf(m[q], x);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        for (int o = 32; o > 0; o >>= 1) m[q] = fmaxf(m[q], __shfl_xor(m[q], o, 64));
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) atomicMax(out + q, __float_as_uint(m[q]));
+        if (bad) atomicOr(out + 4, 1u);
+    }
+}
+
+template <typename T> __global__ void k_fill(T *__restrict__ p, int n, T v) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// Multiplies the stored values of each field by a power of two (exact while in range).
+template <typename S>
+__global__ void __launch_bounds__(256) k_rescale(FieldSet<S> f, long long n, Scl<float> fac) {
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            S *a = fld(f, q);
+            a[c] = (S)((float)a[c] * fac.s[q]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Point sources (functions3D.cuh:1039-1047): p[cell] += w dt c^2 sval amp rho A_s / h^3
+// with amp = u_s[nn] (PCS / Ricker) or sin(2 pi f nn dt/2) (sine). Sources sharing a cell
+// are merged on the host, so the update is race-free.
+// ------------------------------------------------------------------------------------
+template <typename S, typename R>
+__global__ void k_sources(S *__restrict__ p, const int *__restrict__ cell, const double *__restrict__ sval,
+                          const double *__restrict__ freq, const double *__restrict__ u_s, long long us_stride,
+                          int n_src, int opt, long long nn, double w, double dt, double c, double rho, double As, double h,
+                          const int *__restrict__ pidx, double *__restrict__ sh_p, R scale, R inv_scale) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_src) return;
+    typedef Conv<S, R> CV;
+    (void)freq;
+    const double amp = u_s[(long long)s * us_stride + nn];   // shared waveform (stride 0) or per-source table
+    const double add = w * dt * c * c * sval[s] * amp * rho * As / (h * h * h);
+    const int C = cell[s];
+    const int q = pidx ? pidx[C] : -1;
+    if (q >= 0) {                        // FP64 shadow of a reduced-precision field
+        const double v = sh_p[q] + add;
+        sh_p[q] = v;
+        p[C] = Conv<S, double>::st(v, (double)inv_scale);
+    } else {
+        const R cur = CV::ld(p[C], scale);
+        p[C] = CV::st((R)((double)cur + add), inv_scale);
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Immersed boundary + ADE on the packed boundary list (functions3D.cuh:462-565,
+// ADE_solver :426-459). Probe values come from the stage INPUT fields; the blend
+// q = (1-chi) q + chi q_b acts on the stage OUTPUT. All arithmetic is FP64.
+// ADE stage (a,b,c): x_left = (a x_mid + b (x_right + dt xdot(x_right))) / c.
+// ------------------------------------------------------------------------------------
+template <typename G> struct Bnd {
+    int n;
+    const int *cell;        // grid index of the boundary cell
+    const int *base;        // grid index of the trilinear base corner of the probe point
+    const int *mat;         // bits 0-7: material index (oid - 1); bit 8: probe on a cell centre
+    const G *geo;           // [8*n]: sx, sy, sz, nx, ny, nz, chi, pad (FP64 for FP64 fields)
+};
+
+struct Ade {
+    int K, L;
+    const double *Rinf, *A, *zeta, *B, *C, *alpha, *beta;
+    const int *Keff, *Leff;   // poles that can contribute per material
+};
+
+template <typename S>
+__device__ inline double tri_probe(const S *__restrict__ f, int base, int sx_, double sx, double sy, double sz,
+                                   int nx, long long nxy, double scale) {
+#pragma clang fp contract(off)
+    typedef Conv<S, typename std::conditional<std::is_same<S, double>::value, double, float>::type> CV;
+    (void)sx_;
+    const double f000 = (double)CV::ld(f[base], scale);
+    const double f100 = (double)CV::ld(f[base + 1], scale);
+    const double f010 = (double)CV::ld(f[base + nx], scale);
+    const double f110 = (double)CV::ld(f[base + nx + 1], scale);
+    const double f001 = (double)CV::ld(f[base + nxy], scale);
+    const double f101 = (double)CV::ld(f[base + nxy + 1], scale);
+    const double f011 = (double)CV::ld(f[base + nxy + nx], scale);
+    const double f111 = (double)CV::ld(f[base + nxy + nx + 1], scale);
+    const double c00 = f000 * (1 - sx) + f100 * sx;
+    const double c10 = f010 * (1 - sx) + f110 * sx;
+    const double c01 = f001 * (1 - sx) + f101 * sx;
+    const double c11 = f011 * (1 - sx) + f111 * sx;
+    const double c0 = c00 * (1 - sy) + c10 * sy, c1 = c01 * (1 - sy) + c11 * sy;
+    return c0 * (1 - sz) + c1 * sz;
+}
+
+// ADE state buffers of one stage: x_left = (a x_mid + b (x_right + dt xdot(x_right))) / c.
+struct AdeBufs {
+    double *phi_l;
+    const double *phi_m, *phi_r;
+    double *p1_l;
+    const double *p1_m, *p1_r;
+    double *p2_l;
+    const double *p2_m, *p2_r;
+};
+
+template <typename S, typename G>
+__device__ inline void boundary_cell(int b, const Grid &g, const FieldSet<S> &in, const FieldSet<S> &out,
+                                     const Bnd<G> &bd, const Ade &ade, double rho, double c, double dt, double as,
+                                     double bs, double cs, const AdeBufs &ab, const Scl<double> &sc) {
+#pragma clang fp contract(off)
+    double *__restrict__ phi_l = ab.phi_l, *__restrict__ p1_l = ab.p1_l, *__restrict__ p2_l = ab.p2_l;
+    const double *__restrict__ phi_m = ab.phi_m, *__restrict__ phi_r = ab.phi_r;
+    const double *__restrict__ p1_m = ab.p1_m, *__restrict__ p1_r = ab.p1_r;
+    const double *__restrict__ p2_m = ab.p2_m, *__restrict__ p2_r = ab.p2_r;
+    typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type R;
+    typedef Conv<S, R> CV;
+    const int C = bd.cell[b], base = bd.base[b], mw = bd.mat[b];
+    const int m = mw & 0xFF;
+    const G *q = bd.geo + 8 * (long long)b;
+    const double sx = q[0], sy = q[1], sz = q[2], nx = q[3], ny = q[4], nz = q[5], chi = q[6];
+    double pD, uD, vD, wD;
+    if (mw & 0x100) {
+        // probe exactly on a cell centre (all trilinear fractions 0): the interpolant is
+        // that cell's value, bit for bit
+        pD = (double)CV::ld(in.p[base], (R)sc.s[0]);
+        uD = (double)CV::ld(in.u[base], (R)sc.s[1]);
+        vD = (double)CV::ld(in.v[base], (R)sc.s[2]);
+        wD = (double)CV::ld(in.w[base], (R)sc.s[3]);
+    } else {
+        pD = tri_probe(in.p, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[0]);
+        uD = tri_probe(in.u, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[1]);
+        vD = tri_probe(in.v, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[2]);
+        wD = tri_probe(in.w, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[3]);
+    }
+    const double un = uD * nx + vD * ny + wD * nz;
+    const double utx = uD - un * nx, uty = vD - un * ny, utz = wD - un * nz;
+    const double Z0 = rho * c;
+    const double wout = pD + Z0 * un;
+    // ADE (vector-fitted admittance): real poles phi, complex pairs (psi1, psi2)
+    double sumA = 0.0, sumBC = 0.0;
+    const int Ke = ade.Keff[m], Le = ade.Leff[m];
+    for (int k = 0; k < Ke; ++k) {
+        const int mi = m * ade.K + k;
+        const long long s = (long long)b * ade.K + k;
+        const double xr = phi_r[s];
+        const double Lphi = wout - ade.zeta[mi] * xr;
+        double xl = as * phi_m[s] + bs * (xr + dt * Lphi);
+        xl /= cs;
+        phi_l[s] = xl;
+        sumA += ade.A[mi] * xl;
+    }
+    for (int l = 0; l < Le; ++l) {
+        const int mi = m * ade.L + l;
+        const long long s = (long long)b * ade.L + l;
+        const double y1 = p1_r[s], y2 = p2_r[s];
+        const double L1 = wout - (ade.alpha[mi] * y1 + ade.beta[mi] * y2);
+        const double L2 = ade.beta[mi] * y1 - ade.alpha[mi] * y2;
+        double z1 = as * p1_m[s] + bs * (y1 + dt * L1);
+        z1 /= cs;
+        double z2 = as * p2_m[s] + bs * (y2 + dt * L2);
+        z2 /= cs;
+        p1_l[s] = z1;
+        p2_l[s] = z2;
+        sumBC += ade.B[mi] * z1 + ade.C[mi] * z2;
+    }
+    const double win = ade.Rinf[m] * wout + sumA + sumBC;
+    const double pb = 0.5 * (win + wout);
+    const double unb = (wout - win) / (2.0 * Z0);
+    const double ub = unb * nx + utx, vb = unb * ny + uty, wb = unb * nz + utz;
+    const double u_o = (double)CV::ld(out.u[C], (R)sc.s[1]);
+    const double v_o = (double)CV::ld(out.v[C], (R)sc.s[2]);
+    const double w_o = (double)CV::ld(out.w[C], (R)sc.s[3]);
+    const double p_o = (double)CV::ld(out.p[C], (R)sc.s[0]);
+    out.u[C] = CV::st((R)((1.0 - chi) * u_o + chi * ub), (R)sc.is[1]);
+    out.v[C] = CV::st((R)((1.0 - chi) * v_o + chi * vb), (R)sc.is[2]);
+    out.w[C] = CV::st((R)((1.0 - chi) * w_o + chi * wb), (R)sc.is[3]);
+    out.p[C] = CV::st((R)((1.0 - chi) * p_o + chi * pb), (R)sc.is[0]);
+}
+
+template <typename S, typename G>
+__global__ void __launch_bounds__(256)
+k_boundary(Grid g, FieldSet<S> in, FieldSet<S> out, Bnd<G> bd, Ade ade, double rho, double c, double dt,
+           double as, double bs, double cs, AdeBufs ab, Scl<double> sc) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= bd.n) return;
+    boundary_cell<S, G>(b, g, in, out, bd, ade, rho, c, dt, as, bs, cs, ab, sc);
+}
+
+// Receiver gather after a step: ir[r*stride + n] = p(cell_r), vel[(r*stride + n)*3 + a].
+template <typename S>
+__global__ void k_receivers(FieldSet<S> f, const int *__restrict__ cell, int n_rec, double *__restrict__ ir,
+                            double *__restrict__ vel, long long stride, long long n, Scl<double> sc) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rec) return;
+    typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type R;
+    typedef Conv<S, R> CV;
+    const int C = cell[r];
+    ir[r * stride + n] = (double)CV::ld(f.p[C], (R)sc.s[0]);
+    if (vel) {
+        double *o = vel + 3 * (r * stride + n);
+        o[0] = (double)CV::ld(f.u[C], (R)sc.s[1]);
+        o[1] = (double)CV::ld(f.v[C], (R)sc.s[2]);
+        o[2] = (double)CV::ld(f.w[C], (R)sc.s[3]);
+    }
+}
+
+}  // namespace fasesdev

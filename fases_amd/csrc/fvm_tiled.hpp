@@ -70,7 +70,6 @@ __device__ inline void weno_pair(R a, R b, R c, R d, R e, R &A, R &B) {
 namespace tile {
 constexpr int TXO = 62;         // output columns per block
 constexpr int LX = 68;          // LDS columns: x0-3 .. x0+64
-constexpr int TZ = 64;          // planes per block
 }  // namespace tile
 
 template <typename R, int NW> struct TileLds {
@@ -109,23 +108,30 @@ __device__ inline R shfl_next(R x) { return __shfl_down(x, 1, 64); }   // value 
 template <typename R>
 __device__ inline R shfl_prev(R x) { return __shfl_up(x, 1, 64); }     // value of lane - 1
 
-// One block = 64 x NW threads; outputs 62 columns x (NW-2) rows x TZ planes.
+// Field pointers of one tiled stage launch: input set, base set U0, output set.
+template <typename S> struct TileFields {
+    const S *ip, *iu, *iv, *iw;
+    const S *bp, *bu, *bv, *bw;
+    S *op, *ou, *ov, *ow;
+};
+
+// One tile = 64 x NW threads; outputs 62 columns x (NW-2) rows x tz planes of tile
+// (bx, by, bz) of an Nx x Ny x Nz grid.
 template <typename S, typename R, int STAGE, int NW>
-__global__ void __launch_bounds__(64 * NW)
-k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restrict__ iu, const S *__restrict__ iv,
-             const S *__restrict__ iw, const S *__restrict__ bp, const S *__restrict__ bu, const S *__restrict__ bv,
-             const S *__restrict__ bw, S *__restrict__ op, S *__restrict__ ou, S *__restrict__ ov,
-             S *__restrict__ ow, const uint8_t *__restrict__ upd, PmlSet pml, TileK<R> K, R dt, R s_in,
-             R s_base, R is_out) {
+__device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, const TileFields<S> &F,
+                                 const uint8_t *__restrict__ upd, const PmlSet &pml, const TileK<R> &K, R dt,
+                                 const Scl<R> &sc, int tz, int bx, int by, int bz) {
     using namespace tile;
     constexpr int LY = NW + 4, TYO = NW - 2;
-    __shared__ TileLds<R, NW> L;
     typedef Conv<S, R> CV;
+    const S *__restrict__ ip = F.ip, *__restrict__ iu = F.iu, *__restrict__ iv = F.iv, *__restrict__ iw = F.iw;
+    const S *__restrict__ bp = F.bp, *__restrict__ bu = F.bu, *__restrict__ bv = F.bv, *__restrict__ bw = F.bw;
+    S *__restrict__ op = F.op, *__restrict__ ou = F.ou, *__restrict__ ov = F.ov, *__restrict__ ow = F.ow;
     const int lx = threadIdx.x, wy = threadIdx.y;
     const int tid = wy * 64 + lx;
-    const int x0 = 3 + TXO * blockIdx.x, j0 = 3 + TYO * blockIdx.y, k0 = 3 + TZ * blockIdx.z;
+    const int x0 = 3 + TXO * bx, j0 = 3 + TYO * by, k0 = 3 + tz * bz;
     const long long nxy = (long long)Nx * Ny;
-    const int kend = min(k0 + TZ, Nz - 3);
+    const int kend = min(k0 + tz, Nz - 3);
     const int i = x0 - 1 + lx, j = j0 - 1 + wy;
     const int ic = min(i, Nx - 1), jc = min(j, Ny - 1);               // clamped for loads
     const unsigned col = (unsigned)(ic + Nx * jc);                   // offset within a plane
@@ -158,6 +164,7 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
         }
     }
     const S *hsrc = (h_kind == 0) ? ip : (h_kind == 1 ? iu : iv);
+    const R hsc = (h_kind == 0) ? sc.s[0] : (h_kind == 1 ? sc.s[1] : sc.s[2]);
     R *hdst = (h_kind == 0) ? &L.p[h_r][h_c] : (h_kind == 1 ? &L.u[h_r][h_c] : &L.v[h_r][h_c]);
 
     // z pipeline: window p/w at k-1..k+3 (index 0..4), carried A-state of cell k and
@@ -169,8 +176,8 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
             const long long pq = nxy * (long long)(k0 - 3 + q);
-            p7[q] = CV::ld((ip + pq)[col], s_in);
-            w7[q] = CV::ld((iw + pq)[col], s_in);
+            p7[q] = CV::ld((ip + pq)[col], sc.s[0]);
+            w7[q] = CV::ld((iw + pq)[col], sc.s[3]);
         }
         R pAm, wAm, pB0, wB0, tmp0, tmp1;
         weno_pair(p7[0], p7[1], p7[2], p7[3], p7[4], pAm, tmp0);   // cell k0-1
@@ -185,10 +192,10 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
     R nu, nv, np = 0, nh = 0;
     {
         const long long pl = nxy * (long long)k0;
-        nu = CV::ld((iu + pl)[col], s_in);
-        nv = CV::ld((iv + pl)[col], s_in);
-        if (!inner_row) np = CV::ld((ip + pl)[col], s_in);
-        if (h_kind >= 0) nh = CV::ld((hsrc + pl)[h_off], s_in);
+        nu = CV::ld((iu + pl)[col], sc.s[1]);
+        nv = CV::ld((iv + pl)[col], sc.s[2]);
+        if (!inner_row) np = CV::ld((ip + pl)[col], sc.s[0]);
+        if (h_kind >= 0) nh = CV::ld((hsrc + pl)[h_off], hsc);
     }
 
     for (int k = k0; k < kend; ++k) {
@@ -204,10 +211,10 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
         // ---- prefetch the next plane
         if (k + 1 < kend) {
             const long long pn = pl + nxy;
-            nu = CV::ld((iu + pn)[col], s_in);
-            nv = CV::ld((iv + pn)[col], s_in);
-            if (!inner_row) np = CV::ld((ip + pn)[col], s_in);
-            if (h_kind >= 0) nh = CV::ld((hsrc + pn)[h_off], s_in);
+            nu = CV::ld((iu + pn)[col], sc.s[1]);
+            nv = CV::ld((iv + pn)[col], sc.s[2]);
+            if (!inner_row) np = CV::ld((ip + pn)[col], sc.s[0]);
+            if (h_kind >= 0) nh = CV::ld((hsrc + pn)[h_off], hsc);
         }
         // ---- y states of every row (incl. the two outer rows)
         {
@@ -249,8 +256,8 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
             R pnew = 0, wnew = 0;
             if (k + 1 < kend) {
                 const long long p4 = nxy * (long long)(k + 4);
-                pnew = CV::ld((ip + p4)[col], s_in);
-                wnew = CV::ld((iw + p4)[col], s_in);
+                pnew = CV::ld((ip + p4)[col], sc.s[0]);
+                wnew = CV::ld((iw + p4)[col], sc.s[3]);
             }
             if (out_lane && (upd + pl)[col] == 1) {
                 const R sp = -FxP_p + FxM_p - FyP_p + FyM_p - FzP_p + FzP;
@@ -263,8 +270,8 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
                     po = fma_s(K.dth, sp, pc);
                     wgt = R(1);
                 } else {
-                    const R p0 = CV::ld((bp + pl)[col], s_base), u0 = CV::ld((bu + pl)[col], s_base);
-                    const R v0 = CV::ld((bv + pl)[col], s_base), w0 = CV::ld((bw + pl)[col], s_base);
+                    const R p0 = CV::ld((bp + pl)[col], sc.s[0]), u0 = CV::ld((bu + pl)[col], sc.s[1]);
+                    const R v0 = CV::ld((bv + pl)[col], sc.s[2]), w0 = CV::ld((bw + pl)[col], sc.s[3]);
                     if (STAGE == 1) {
                         uo = fma(R(0.75), u0, R(0.25) * fma_s(K.dth, Du, uc));
                         vo = fma(R(0.75), v0, R(0.25) * fma_s(K.dth, Dv, vc));
@@ -283,10 +290,10 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
                     pml_apply<R, STAGE == 2>(pml, i, j, k, Nx, Ny, po, uo, vo, wo, (-FxP_p + FxM_p) * K.ih,
                                              (-FyP_p + FyM_p) * K.ih, (-FzP_p + FzP) * K.ih, Du * K.ih, Dv * K.ih,
                                              Dw * K.ih, wgt * dt);
-                (ou + pl)[col] = CV::st(uo, is_out);
-                (ov + pl)[col] = CV::st(vo, is_out);
-                (ow + pl)[col] = CV::st(wo, is_out);
-                (op + pl)[col] = CV::st(po, is_out);
+                (ou + pl)[col] = CV::st(uo, sc.is[1]);
+                (ov + pl)[col] = CV::st(vo, sc.is[2]);
+                (ow + pl)[col] = CV::st(wo, sc.is[3]);
+                (op + pl)[col] = CV::st(po, sc.is[0]);
             }
             // advance the z pipeline
             pA = pAn; wA = wAn; FzP = FzP_p; FzW = FzP_w;
@@ -294,6 +301,14 @@ k_flux_tiled(int Nx, int Ny, int Nz, const S *__restrict__ ip, const S *__restri
             wz[0] = wz[1]; wz[1] = wz[2]; wz[2] = wz[3]; wz[3] = wl4; wz[4] = wnew;
         }
     }
+}
+
+template <typename S, typename R, int STAGE, int NW>
+__global__ void __launch_bounds__(64 * NW)
+k_flux_tiled(int Nx, int Ny, int Nz, TileFields<S> F, const uint8_t *__restrict__ upd, PmlSet pml, TileK<R> K, R dt,
+             Scl<R> sc, int tz) {
+    __shared__ TileLds<R, NW> L;
+    flux_tile<S, R, STAGE, NW>(L, Nx, Ny, Nz, F, upd, pml, K, dt, sc, tz, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 }  // namespace fasesdev

@@ -56,6 +56,7 @@ int fases_params_default(fases_params *p) {
     p->fp16_rescale_every = 100;
     p->use_graph = 0;
     p->kernel_variant = 0;
+    p->tile_z = 0;
     return FASES_OK;
 }
 

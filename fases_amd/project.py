@@ -55,7 +55,7 @@ def load_project(scene: str, config: str, src: str, rec: str) -> Workload:
     Nt = int(f32(t_max / dt)) + 1
     src_opt = int(conf[15]) if conf.size > 15 else 0
     glob_f = float(conf[16]) if conf.size > 16 else (25.0 if src_opt == 0 else 0.5 * float(f_cut))
-    prec = {0: "fp64", 1: "fp32"}[int(ext.get("precision", 1))]
+    prec = {0: "fp64", 1: "fp32", 2: "fp16"}[int(ext.get("precision", 1))]
 
     sc = Scene.from_obj(scene, float(h))
     cfg_dir = Path(config).resolve().parent

@@ -253,6 +253,7 @@ class SolverParams:
     fp16_rescale_every: int = 100
     use_graph: bool = False
     kernel_variant: int = 0             # 0 auto, 1 per-cell (reference order), 2 tiled
+    tile_z: int = 0                     # tiled kernel z planes per block; 0 = autotune
 
     def to_c(self) -> _lib.fases_params:
         p = _lib.fases_params()
@@ -269,6 +270,7 @@ class SolverParams:
         p.fp16_rescale_every = self.fp16_rescale_every
         p.use_graph = int(self.use_graph)
         p.kernel_variant = int(self.kernel_variant)
+        p.tile_z = int(self.tile_z)
         return p
 
 
@@ -349,3 +351,22 @@ class Solver:
         a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
         self._chk(self._lib.fases_info(self._ctx, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         return {"n_cells": a.value, "n_bnd": b.value, "n_upd": c.value, "dev_bytes": d.value}
+
+    def tile_config(self) -> tuple:
+        """(columns, rows, z planes) per block of the tiled flux kernel (z = 0 before tuning)."""
+        x, y, z = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self._lib.fases_tile_config(self._ctx, C.byref(x), C.byref(y), C.byref(z)))
+        return x.value, y.value, z.value
+
+
+def batch_run(solvers: Sequence[Solver], nsteps: Sequence[int]):
+    """Advances the solvers together (fases_batch_run); returns ([ir_j [n_rec_j, nsteps_j]], elapsed_ms)."""
+    n = len(solvers)
+    ctxs = (C.c_void_p * n)(*[s._ctx.value for s in solvers])
+    ns = np.ascontiguousarray(nsteps, np.int64)
+    irs = [np.zeros((s.n_rec, int(k))) for s, k in zip(solvers, ns)]
+    ptrs = (C.POINTER(C.c_double) * n)(*[_ptr(a, C.c_double) for a in irs])
+    el = C.c_double(0)
+    rc = load().fases_batch_run(ctxs, n, _ptr(ns, C.c_int64), ptrs, C.byref(el))
+    check(rc, solvers[0]._ctx)
+    return irs, el.value

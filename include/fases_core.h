@@ -71,6 +71,8 @@ typedef struct fases_params {
                                1 per-cell kernel (reference operation order, no FMA
                                contraction: bit-faithful to an FP64 CPU evaluation);
                                2 tiled kernel (faces reconstructed once, LDS/shuffles) */
+    int    tile_z;          /* tiled kernel: z planes per block; 0 = autotune at the
+                               first fases_run (times 16, 32, 64, 128 on the real grid) */
 } fases_params;
 
 /* Fills *p with the reference defaults (FP64, device 0, PML depth 20, ...). */
@@ -192,7 +194,22 @@ int fases_reset(fases_ctx *ctx);
 int fases_run(fases_ctx *ctx, int64_t nsteps, double *ir, double *vel,
               double *elapsed_ms, double *kernel_ms);
 
+/* Advances n independent contexts (jobs) together: nsteps[j] steps for job j, each
+ * stage of all jobs in one launch per kernel kind, the step loop captured into
+ * hipGraphs and replayed; jobs drop out as their step counts run out. ir[j]
+ * (nullable) receives job j's receiver pressures [n_rec_j * nsteps[j]], exactly as
+ * fases_run would give them. Jobs must share the device and use FP32 storage.
+ * *elapsed_ms (nullable) = HIP-event time of the whole batch.
+ * Replaces running the reference executable once per job (one process per source,
+ * common/gui.py:11137-11168, each stepping main3D.cu:1280-1320 alone). */
+int fases_batch_run(fases_ctx **ctx, int n, const int64_t *nsteps, double **ir, double *elapsed_ms);
+
 /* Sizes: padded cells, boundary cells, room (updated) cells, device bytes in use. */
+/* Tile of the tiled flux kernel in use: output columns x rows x z planes per block
+   (*tile_z = 0 before the first fases_run when autotuning). No counterpart in the
+   reference (one thread per cell there); reported so runs can state their launch shape. */
+int fases_tile_config(const fases_ctx *ctx, int *tile_x, int *tile_y, int *tile_z);
+
 int fases_info(const fases_ctx *ctx, int64_t *n_cells, int64_t *n_bnd, int64_t *n_upd,
                int64_t *dev_bytes);
 
