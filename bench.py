@@ -8,7 +8,11 @@ full SSP-RK3 step (3 flux stages + sources + immersed boundary) over the room.
 Prints ONE JSON line: Mcell-updates/s (room cells x steps / HIP-event time of the
 step loop), the flux-stage kernel's achieved algorithmic HBM bandwidth vs the 8 TB/s
 peak, the IR relative L2 of this run vs the FP64 path on the same room and window,
-and the FP64 CPU oracle timed on a bounded sample on this host.
+and the FP64 CPU oracle timed on a bounded sample on this host. Beside the headline:
+the same room with FP16 storage (per-field power-of-two scales, FP32 math) and its IR
+error vs the FP64 and FP32 runs, and the batched benchmark (32 seeded rooms of 0.9-7.9 M
+cells, 1500-4000 steps each, FP32) run as one batch vs the same jobs run one after the
+other.
 """
 from __future__ import annotations
 
@@ -43,6 +47,8 @@ def main() -> int:
     ap.add_argument("--room", default="512,384,256")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline sample (0: skip)")
     ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=32, help="jobs of the batched benchmark (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the FVM core has no multi-GPU path in this build "
@@ -67,6 +73,7 @@ def main() -> int:
     t1 = time.time()
     ir_t, _, ms, kms = sv.run(args.steps, kernel_timing=True)
     wall = time.time() - t1
+    tile = sv.tile_config()
     sv.close()
     cells = wl.room_cells
     mcells = cells * args.steps / (ms * 1e-3) / 1e6
@@ -86,14 +93,63 @@ def main() -> int:
         except Exception:
             traffic = None
 
-    ir_err = None
+    from fases_amd.metrics import decay_error_db, rel_l2
+    ir = np.concatenate([ir_w, ir_t], axis=1)
+    ir_err, ir64 = None, None
     if not args.no_fp64_check and args.precision != "fp64":
         sv64 = W.make_solver(wl, "fp64")
         ir64 = np.concatenate([sv64.run(args.warmup)[0], sv64.run(args.steps)[0]], axis=1)
         sv64.close()
-        ir = np.concatenate([ir_w, ir_t], axis=1)
-        ir_err = float(np.linalg.norm(ir - ir64) / max(np.linalg.norm(ir64), 1e-300))
+        ir_err = rel_l2(ir, ir64)
         log("IR rel L2 vs FP64 path over %d steps: %.3e (|IR|max %.3e)" % (total, ir_err, np.abs(ir64).max()))
+
+    fp16 = None
+    if not args.no_fp16 and args.precision == "fp32":
+        sv16 = W.make_solver(wl, "fp16")
+        ir16w, _, _, _ = sv16.run(args.warmup)
+        ir16t, _, ms16, kms16 = sv16.run(args.steps, kernel_timing=True)
+        sv16.close()
+        ir16 = np.concatenate([ir16w, ir16t], axis=1)
+        fp16 = {"value": round(cells * args.steps / (ms16 * 1e-3) / 1e6, 2), "unit": "Mcell-updates/s",
+                "ms_per_step": round(ms16 / args.steps, 5),
+                "flux_roofline_frac": round(info["n_upd"] * 32 * 2 / 3.0 / (kms16 * 1e-3 / launches) / 1e9
+                                            / HBM_PEAK_GBS, 4),
+                "ir_rel_l2_vs_fp32": rel_l2(ir16, ir),
+                "ir_rel_l2_vs_fp64": rel_l2(ir16, ir64) if ir64 is not None else None,
+                "decay_err_db_vs_fp64": decay_error_db(ir16, ir64) if ir64 is not None else None,
+                "decay_err_db_vs_fp32": decay_error_db(ir16, ir),
+                "rescale_every": wl.params.fp16_rescale_every,
+                "note": "same room and window; decay error = max |EDC difference| down to -40 dB"}
+        log("fp16 storage: %.0f Mcell/s, IR rel L2 vs FP64 %s, vs FP32 %.3e"
+            % (fp16["value"], fp16["ir_rel_l2_vs_fp64"], fp16["ir_rel_l2_vs_fp32"]))
+
+    batch = None
+    if args.batch_jobs > 0:
+        from fases_amd.solver import batch_run
+        jobs = W.batch_jobs(args.batch_jobs)
+        work = sum(int(np.prod(j.room)) * j.nsteps for j in jobs)
+        seq_ms, seq_ir = 0.0, []
+        for j in jobs:
+            s1 = W.make_solver(j)
+            r, _, el, _ = s1.run(j.nsteps)
+            s1.close()
+            seq_ms += el
+            seq_ir.append(r)
+        log("batched benchmark: %d jobs sequential %.0f ms" % (len(jobs), seq_ms))
+        svs = [W.make_solver(j) for j in jobs]
+        b_ir, b_ms = batch_run(svs, [j.nsteps for j in jobs])
+        for s1 in svs:
+            s1.close()
+        same = all(np.array_equal(a, b) for a, b in zip(b_ir, seq_ir))
+        batch = {"jobs": len(jobs), "cell_steps": work, "batched_ms": round(b_ms, 1),
+                 "sequential_ms": round(seq_ms, 1), "speedup": round(seq_ms / b_ms, 3),
+                 "batched_value": round(work / (b_ms * 1e-3) / 1e6, 1),
+                 "sequential_value": round(work / (seq_ms * 1e-3) / 1e6, 1), "unit": "Mcell-updates/s",
+                 "irs_bitwise_equal": bool(same),
+                 "workload": "seed 1003: rooms U[96..256]x[96..192]x[96..160] cells, dx 0.02, 1-3 wall types, "
+                             "Ricker f_p U[500,4000] Hz, 1500-4000 steps, 2 receivers, FP32; sequential = "
+                             "fases_run per job (autotuned tile), batched = fases_batch_run (hipGraph replay)"}
+        log("batched %.0f ms vs sequential %.0f ms: x%.2f, IRs identical %s" % (b_ms, seq_ms, seq_ms / b_ms, same))
 
     cpu = None
     if args.cpu_steps > 0:
@@ -125,12 +181,15 @@ def main() -> int:
                                "(3 types, K=4 L=2), Ricker 2 kHz, 8 receivers" % ("x".join(map(str, room)),
                                                                                    args.precision.upper()),
                    "room_cells": cells, "padded_cells": info["n_cells"], "boundary_cells": info["n_bnd"],
-                   "updated_cells": info["n_upd"], "parallelism": "single GPU"},
+                   "updated_cells": info["n_upd"], "parallelism": "single GPU",
+                   "tile": "%dx%dx%d" % tile},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "flux stage (WENO5-Z + Godunov + SSP-RK3 + cPML)",
                      "alg_bytes_per_launch": int(alg_bytes_launch)},
         "ir_rel_l2_vs_fp64": ir_err,
+        "fp16_storage": fp16,
+        "batched": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -108,6 +108,15 @@ __device__ inline R shfl_next(R x) { return __shfl_down(x, 1, 64); }   // value 
 template <typename R>
 __device__ inline R shfl_prev(R x) { return __shfl_up(x, 1, 64); }     // value of lane - 1
 
+// Element at a 32-bit byte offset from a base pointer (the offset of a lane within a
+// plane is computed once; the plane base changes per plane).
+template <typename T> __device__ inline T &at(T *base, unsigned boff) {
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff);
+}
+template <typename T> __device__ inline const T &at(const T *base, unsigned boff) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+
 // Field pointers of one tiled stage launch: input set, base set U0, output set.
 template <typename S> struct TileFields {
     const S *ip, *iu, *iv, *iw;
@@ -127,7 +136,8 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     const S *__restrict__ ip = F.ip, *__restrict__ iu = F.iu, *__restrict__ iv = F.iv, *__restrict__ iw = F.iw;
     const S *__restrict__ bp = F.bp, *__restrict__ bu = F.bu, *__restrict__ bv = F.bv, *__restrict__ bw = F.bw;
     S *__restrict__ op = F.op, *__restrict__ ou = F.ou, *__restrict__ ov = F.ov, *__restrict__ ow = F.ow;
-    const int lx = threadIdx.x, wy = threadIdx.y;
+    // blockDim.x is one wavefront, so the row index is uniform per wave: keep it scalar
+    const int lx = threadIdx.x, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int tid = wy * 64 + lx;
     const int x0 = 3 + TXO * bx, j0 = 3 + TYO * by, k0 = 3 + tz * bz;
     const long long nxy = (long long)Nx * Ny;
@@ -135,6 +145,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     const int i = x0 - 1 + lx, j = j0 - 1 + wy;
     const int ic = min(i, Nx - 1), jc = min(j, Ny - 1);               // clamped for loads
     const unsigned col = (unsigned)(ic + Nx * jc);                   // offset within a plane
+    const unsigned cb = col * (unsigned)sizeof(S);                   // ... in bytes
     const bool inner_row = (wy >= 1 && wy <= NW - 2);
     const bool out_lane = (lx >= 1 && lx <= TXO) && i < Nx - 3 && j < Ny - 3 && inner_row;
 
@@ -169,15 +180,15 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
 
     // z pipeline: window p/w at k-1..k+3 (index 0..4), carried A-state of cell k and
     // flux through k-1/2
-    R pz[5], wz[5];
+    R pz[5] = {0, 0, 0, 0, 0}, wz[5] = {0, 0, 0, 0, 0};
     R pA = 0, wA = 0, FzP = 0, FzW = 0;
     if (inner_row) {
         R p7[7], w7[7];
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
             const long long pq = nxy * (long long)(k0 - 3 + q);
-            p7[q] = CV::ld((ip + pq)[col], sc.s[0]);
-            w7[q] = CV::ld((iw + pq)[col], sc.s[3]);
+            p7[q] = CV::ld(at(ip + pq, cb), sc.s[0]);
+            w7[q] = CV::ld(at(iw + pq, cb), sc.s[3]);
         }
         R pAm, wAm, pB0, wB0, tmp0, tmp1;
         weno_pair(p7[0], p7[1], p7[2], p7[3], p7[4], pAm, tmp0);   // cell k0-1
@@ -192,10 +203,10 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     R nu, nv, np = 0, nh = 0;
     {
         const long long pl = nxy * (long long)k0;
-        nu = CV::ld((iu + pl)[col], sc.s[1]);
-        nv = CV::ld((iv + pl)[col], sc.s[2]);
-        if (!inner_row) np = CV::ld((ip + pl)[col], sc.s[0]);
-        if (h_kind >= 0) nh = CV::ld((hsrc + pl)[h_off], hsc);
+        nu = CV::ld(at(iu + pl, cb), sc.s[1]);
+        nv = CV::ld(at(iv + pl, cb), sc.s[2]);
+        if (!inner_row) np = CV::ld(at(ip + pl, cb), sc.s[0]);
+        if (h_kind >= 0) nh = CV::ld((hsrc + pl)[h_off], hsc);   // per-lane field: no uniform base
     }
 
     for (int k = k0; k < kend; ++k) {
@@ -211,9 +222,9 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         // ---- prefetch the next plane
         if (k + 1 < kend) {
             const long long pn = pl + nxy;
-            nu = CV::ld((iu + pn)[col], sc.s[1]);
-            nv = CV::ld((iv + pn)[col], sc.s[2]);
-            if (!inner_row) np = CV::ld((ip + pn)[col], sc.s[0]);
+            nu = CV::ld(at(iu + pn, cb), sc.s[1]);
+            nv = CV::ld(at(iv + pn, cb), sc.s[2]);
+            if (!inner_row) np = CV::ld(at(ip + pn, cb), sc.s[0]);
             if (h_kind >= 0) nh = CV::ld((hsrc + pn)[h_off], hsc);
         }
         // ---- y states of every row (incl. the two outer rows)
@@ -239,27 +250,25 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         }
         __syncthreads();
 
+        R pAn = 0, wAn = 0, FzP_p = 0, FzP_w = 0, pnew = 0, wnew = 0;
         if (inner_row) {
             // ---- y faces j+1/2 and j-1/2
             R FyP_p, FyP_v, FyM_p, FyM_v;
             godunov_f(L.ys[0][wy][lx], L.ys[1][wy + 1][lx], L.ys[2][wy][lx], L.ys[3][wy + 1][lx], K, FyP_p, FyP_v);
             godunov_f(L.ys[0][wy - 1][lx], L.ys[1][wy][lx], L.ys[2][wy - 1][lx], L.ys[3][wy][lx], K, FyM_p, FyM_v);
             // ---- z face k+1/2 from the new cell k+1
-            R pAn, pBn, wAn, wBn;
+            R pBn, wBn;
             weno_pair(pz[0], pz[1], pz[2], pz[3], pz[4], pAn, pBn);
             weno_pair(wz[0], wz[1], wz[2], wz[3], wz[4], wAn, wBn);
-            R FzP_p, FzP_w;
             godunov_f(pA, pBn, wA, wBn, K, FzP_p, FzP_w);
             const R wc = wz[1];
             // z window for the next plane: issue the lead loads before the update
-            const R pl4 = pz[4], wl4 = wz[4];
-            R pnew = 0, wnew = 0;
             if (k + 1 < kend) {
                 const long long p4 = nxy * (long long)(k + 4);
-                pnew = CV::ld((ip + p4)[col], sc.s[0]);
-                wnew = CV::ld((iw + p4)[col], sc.s[3]);
+                pnew = CV::ld(at(ip + p4, cb), sc.s[0]);
+                wnew = CV::ld(at(iw + p4, cb), sc.s[3]);
             }
-            if (out_lane && (upd + pl)[col] == 1) {
+            if (out_lane && at(upd + pl, col) == 1) {
                 const R sp = -FxP_p + FxM_p - FyP_p + FyM_p - FzP_p + FzP;
                 const R Du = -FxP_u + FxM_u, Dv = -FyP_v + FyM_v, Dw = -FzP_w + FzW;
                 R uo, vo, wo, po, wgt;
@@ -270,8 +279,8 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
                     po = fma_s(K.dth, sp, pc);
                     wgt = R(1);
                 } else {
-                    const R p0 = CV::ld((bp + pl)[col], sc.s[0]), u0 = CV::ld((bu + pl)[col], sc.s[1]);
-                    const R v0 = CV::ld((bv + pl)[col], sc.s[2]), w0 = CV::ld((bw + pl)[col], sc.s[3]);
+                    const R p0 = CV::ld(at(bp + pl, cb), sc.s[0]), u0 = CV::ld(at(bu + pl, cb), sc.s[1]);
+                    const R v0 = CV::ld(at(bv + pl, cb), sc.s[2]), w0 = CV::ld(at(bw + pl, cb), sc.s[3]);
                     if (STAGE == 1) {
                         uo = fma(R(0.75), u0, R(0.25) * fma_s(K.dth, Du, uc));
                         vo = fma(R(0.75), v0, R(0.25) * fma_s(K.dth, Dv, vc));
@@ -290,16 +299,17 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
                     pml_apply<R, STAGE == 2>(pml, i, j, k, Nx, Ny, po, uo, vo, wo, (-FxP_p + FxM_p) * K.ih,
                                              (-FyP_p + FyM_p) * K.ih, (-FzP_p + FzP) * K.ih, Du * K.ih, Dv * K.ih,
                                              Dw * K.ih, wgt * dt);
-                (ou + pl)[col] = CV::st(uo, sc.is[1]);
-                (ov + pl)[col] = CV::st(vo, sc.is[2]);
-                (ow + pl)[col] = CV::st(wo, sc.is[3]);
-                (op + pl)[col] = CV::st(po, sc.is[0]);
+                at(ou + pl, cb) = CV::st(uo, sc.is[1]);
+                at(ov + pl, cb) = CV::st(vo, sc.is[2]);
+                at(ow + pl, cb) = CV::st(wo, sc.is[3]);
+                at(op + pl, cb) = CV::st(po, sc.is[0]);
             }
-            // advance the z pipeline
-            pA = pAn; wA = wAn; FzP = FzP_p; FzW = FzP_w;
-            pz[0] = pz[1]; pz[1] = pz[2]; pz[2] = pz[3]; pz[3] = pl4; pz[4] = pnew;
-            wz[0] = wz[1]; wz[1] = wz[2]; wz[2] = wz[3]; wz[3] = wl4; wz[4] = wnew;
         }
+        // advance the z pipeline (every row: the outer rows carry zeros, so the window
+        // registers are the same on both sides of the row branch and nothing is copied)
+        pA = pAn; wA = wAn; FzP = FzP_p; FzW = FzP_w;
+        pz[0] = pz[1]; pz[1] = pz[2]; pz[2] = pz[3]; pz[3] = pz[4]; pz[4] = pnew;
+        wz[0] = wz[1]; wz[1] = wz[2]; wz[2] = wz[3]; wz[3] = wz[4]; wz[4] = wnew;
     }
 }
 

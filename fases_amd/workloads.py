@@ -193,6 +193,25 @@ def ade_room(room, h, nsteps, seed, n_rec, n_types=None, K=4, L=2, pml_x_hi=True
                     materials=m, rec_ijk=recs, src_ijk=[src], src_sval=[sval], u_s=u_s)
 
 
+def batch_jobs(n: int = 32, seed: int = 1003, h: float = 0.02) -> list:
+    """Independent rooms of the batched benchmark: room cells U[96..256] x U[96..192] x
+    U[96..160], 1-3 wall materials, a Ricker source with peak U[500, 4000] Hz, U[1500, 4000]
+    steps, 2 receivers, FP32 storage, all drawn from one seed."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for q in range(n):
+        room = (int(rng.integers(96, 257)), int(rng.integers(96, 193)), int(rng.integers(96, 161)))
+        n_types = int(rng.integers(1, 4))
+        f_peak = float(rng.uniform(500.0, 4000.0))
+        steps = int(rng.integers(1500, 4001))
+        sub = int(rng.integers(0, 2 ** 31 - 1))
+        wl = ade_room(room=room, h=h, nsteps=steps, seed=sub, n_rec=2, n_types=n_types, pml_x_hi=False,
+                      f_peak=f_peak, precision="fp32")
+        wl.name = "job%02d_%dx%dx%d" % ((q,) + room)
+        jobs.append(wl)
+    return jobs
+
+
 def config(n: int, **over) -> Workload:
     """Workload of benchmark configuration n (0..2, 4)."""
     if n == 0:

@@ -1,0 +1,109 @@
+"""Batched multi-job loop (fases_batch_run) and the tile autotuner.
+
+Each job of a batch runs the same per-cell arithmetic as a single-context fases_run, so
+the bar is bit-for-bit equality with the single runs (IRs and the final fields, which
+also checks the host-side rotation bookkeeping after odd and even step counts)."""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from fases_amd import _lib
+from fases_amd import workloads as W
+from fases_amd._lib import FasesError
+from fases_amd.solver import batch_run
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def jobs():
+    out = []
+    for room, n, seed, pml in (((40, 30, 24), 61, 1, True), ((52, 26, 30), 40, 2, False), ((30, 34, 20), 75, 3, True)):
+        out.append(W.ade_room(room=room, h=0.02, nsteps=n, seed=seed, n_rec=2, K=2, L=1, pml_cells=6, pml_x_hi=pml,
+                              precision="fp32"))
+    # a sine-driven job and one without sources (initial pulse only)
+    s = out[1]
+    out[1] = dataclasses.replace(s, src_opt=_lib.FASES_SRC_SINE, u_s=None, src_sval=[2.0], src_freq=[700.0])
+    r = W.rigid_box(room=(26, 22, 18), h=0.05, nsteps=33, ic_center=(8, 8, 8), rec=(18, 14, 10))
+    out.append(dataclasses.replace(r, params=dataclasses.replace(r.params, precision="fp32")))
+    return out
+
+
+EXTRA = 5   # steps run after the batch (the workloads' waveforms cover them)
+
+
+def test_batch_equals_single_runs_bitwise():
+    js = jobs()
+    steps = [wl.nsteps - EXTRA for wl in js]
+    announce("batch of %d jobs, steps %s" % (len(js), steps))
+    single_ir, single_f = [], []
+    for wl, n in zip(js, steps):
+        sv = W.make_solver(wl, "fp32")
+        ir, _, _, _ = sv.run(n)
+        single_ir.append(ir)
+        single_f.append(sv.get_fields())
+        sv.close()
+    svs = [W.make_solver(wl, "fp32") for wl in js]
+    irs, ms = batch_run(svs, steps)
+    assert ms > 0
+    for q, sv in enumerate(svs):
+        assert np.array_equal(irs[q], single_ir[q]), q
+        f = sv.get_fields()
+        for a, b in zip(f, single_f[q]):
+            assert np.array_equal(a, b), q
+    # the jobs continue from the batch's end state like a single context would
+    more = [sv.run(EXTRA)[0] for sv in svs]
+    for sv in svs:
+        sv.close()
+    for q, wl in enumerate(js):
+        sv = W.make_solver(wl, "fp32")
+        ir, _, _, _ = sv.run(wl.nsteps)
+        sv.close()
+        assert np.array_equal(ir[:, steps[q]:], more[q]), q
+
+
+def test_tiled_result_independent_of_tile_depth():
+    """Every face is computed by the same operations whichever block holds it, so the
+    z depth of a block changes nothing in the result."""
+    wl = W.ade_room(room=(40, 30, 70), h=0.02, nsteps=30, seed=4, n_rec=3, K=2, L=1, pml_cells=6, precision="fp32")
+    announce("tile depth independence")
+    res = []
+    for tz in (16, 32, 64, 128):
+        w2 = dataclasses.replace(wl, params=dataclasses.replace(wl.params, tile_z=tz))
+        sv = W.make_solver(w2, "fp32")
+        ir, _, _, _ = sv.run(wl.nsteps)
+        assert sv.tile_config()[2] == tz
+        res.append((ir, sv.get_fields()[0]))
+        sv.close()
+    for ir, p in res[1:]:
+        assert np.array_equal(ir, res[0][0]) and np.array_equal(p, res[0][1])
+
+
+def test_autotuner_picks_a_candidate():
+    wl = W.ade_room(room=(40, 30, 24), h=0.02, nsteps=4, seed=1, n_rec=1, K=2, L=1, pml_cells=6, precision="fp32")
+    announce("autotuner")
+    sv = W.make_solver(wl, "fp32")
+    assert sv.tile_config()[2] == 0
+    sv.run(2)
+    tx, ty, tz = sv.tile_config()
+    sv.close()
+    assert (tx, ty) == (62, 14) and tz in (16, 32, 64, 128)
+
+
+def test_batch_rejects_bad_input():
+    wl = jobs()[0]
+    announce("batch errors")
+    a = W.make_solver(wl, "fp32")
+    b = W.make_solver(wl, "fp64")
+    with pytest.raises(FasesError, match="FP32"):
+        batch_run([a, b], [3, 3])
+    with pytest.raises(FasesError, match="twice"):
+        batch_run([a, a], [3, 3])
+    a.close()
+    b.close()
