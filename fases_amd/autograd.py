@@ -1,0 +1,123 @@
+"""torch.autograd.Function over the hand-written adjoint of the FVM run, and an L-BFGS
+recovery of ADE material parameters from receiver IRs.
+
+The differentiable inputs are the material table (per material: R_inf, A[K], zeta[K],
+B[L], C[L], alpha[L], beta[L], the same layout as fases_adjoint's gradient) and the
+source drive amplitudes; the output is the receiver pressure IR [n_rec, nsteps].
+PyTorch only carries the tensors and the optimizer: the forward run, the checkpointed
+reverse sweep and the gradients are the library's (fases_adjoint_forward /
+fases_adjoint). This replaces the reference's gradient-free fitting loops
+(vectFIT.py / fminconFIT.py) with gradients of the full room simulation.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ._lib import FasesError
+from .solver import Materials
+
+FIELDS = ("Rinf", "A", "zeta", "B", "C", "alpha", "beta")
+
+
+def pack(m: Materials) -> np.ndarray:
+    """Materials -> [nMat, 1 + 2K + 4L] in the adjoint's gradient layout."""
+    return np.concatenate([m.Rinf[:, None], m.A, m.zeta, m.B, m.C, m.alpha, m.beta], axis=1)
+
+
+def unpack(theta: np.ndarray, K: int, L: int) -> Materials:
+    t = np.asarray(theta, np.float64)
+    o = [0, 1, 1 + K, 1 + 2 * K, 1 + 2 * K + L, 1 + 2 * K + 2 * L, 1 + 2 * K + 3 * L, 1 + 2 * K + 4 * L]
+    parts = [t[:, o[q]:o[q + 1]] for q in range(7)]
+    return Materials(parts[0][:, 0].copy(), *[p.copy() for p in parts[1:]])
+
+
+class RoomIR(torch.autograd.Function):
+    """ir = RoomIR.apply(theta [nMat, P], sval [n_src], problem). `problem` is a
+    RoomProblem (solver, workload, steps, checkpoint interval). backward() runs the
+    adjoint for the last forward of that solver, so call it before the next forward."""
+
+    @staticmethod
+    def forward(ctx, theta, sval, problem):
+        ir = problem.forward(theta.detach().cpu().numpy(), sval.detach().cpu().numpy())
+        ctx.problem = problem
+        return torch.from_numpy(ir)
+
+    @staticmethod
+    def backward(ctx, grad_ir):
+        gm, gs = ctx.problem.backward(grad_ir.detach().cpu().numpy())
+        return torch.from_numpy(gm), torch.from_numpy(gs), None
+
+
+class RoomProblem:
+    """A loaded solver context and workload with differentiable materials and drives."""
+
+    def __init__(self, solver, workload, nsteps: int | None = None, checkpoint_every: int = 64):
+        self.sv = solver
+        self.wl = workload
+        self.nsteps = workload.nsteps if nsteps is None else nsteps
+        self.every = checkpoint_every
+        self.K, self.L = workload.materials.K, workload.materials.L
+        self.cells = [workload.cell(s) for s in workload.src_ijk]
+
+    def forward(self, theta: np.ndarray, sval: np.ndarray) -> np.ndarray:
+        wl = self.wl
+        self.sv.set_materials(unpack(theta, self.K, self.L))
+        self.sv.set_sources(self.cells, sval, wl.src_opt, wl.u_s, wl.src_freq)
+        self.sv.set_fields(wl.initial_pressure())
+        return self.sv.adjoint_forward(self.nsteps, self.every)
+
+    def backward(self, dj_dir: np.ndarray):
+        gm, gs, _ = self.sv.adjoint(dj_dir)
+        return gm, gs
+
+
+def free_mask(K: int, L: int, n: int) -> np.ndarray:
+    """1 for the fitted entries (R_inf, residues A, B, C, pole real parts zeta, alpha),
+    0 for the pair frequencies beta, which stay at their initial values."""
+    m = np.ones((n, 1 + 2 * K + 4 * L))
+    m[:, 1 + 2 * K + 3 * L:] = 0.0
+    return m
+
+
+def lbfgs_recover(problem: RoomProblem, target_ir: np.ndarray, theta0: np.ndarray, sval0: np.ndarray,
+                  iterations: int = 50, lr: float = 0.1, warm: int = 3, log=None):
+    """Fits theta = theta0 * exp(u) (multiplicative, so signs and scales are kept; the pair
+    frequencies stay fixed) and sval = sval0 * exp(v) to the target IRs by L-BFGS on
+    J = sum (ir - target)^2. Returns (theta, sval, history of J per iteration).
+    The line search starts at lr times the quasi-Newton step for the first `warm`
+    iterations (a full first step can push a wall past |R| = 1, where the room becomes
+    unstable), at the full step afterwards; a trial point whose run goes non-finite
+    counts as a very large misfit so the line search backs off."""
+    th0 = torch.from_numpy(np.asarray(theta0, np.float64))
+    s0 = torch.from_numpy(np.asarray(sval0, np.float64))
+    mask = torch.from_numpy(free_mask(problem.K, problem.L, th0.shape[0]))
+    u = torch.zeros_like(th0, requires_grad=True)
+    v = torch.zeros_like(s0, requires_grad=True)
+    tgt = torch.from_numpy(np.asarray(target_ir, np.float64))
+    opt = torch.optim.LBFGS([u, v], lr=lr, max_iter=1, history_size=20, line_search_fn="strong_wolfe")
+    hist = []
+
+    def closure():
+        opt.zero_grad()
+        try:
+            ir = RoomIR.apply(th0 * torch.exp(u * mask), s0 * torch.exp(v), problem)
+        except FasesError:
+            # unstable trial point: huge misfit, gradient pointing back to the origin
+            big = torch.tensor(1e30, dtype=torch.float64)
+            (big * ((u * mask) ** 2).sum() + big * (v ** 2).sum()).backward()
+            return big
+        J = ((ir - tgt) ** 2).sum()
+        J.backward()
+        return J
+
+    for it in range(iterations):
+        if it == warm:
+            for g in opt.param_groups:
+                g["lr"] = 1.0
+        J = float(opt.step(closure).detach())
+        hist.append(J)
+        if log:
+            log("L-BFGS iteration %d: J = %.6e" % (it, J))
+    with torch.no_grad():
+        return (th0 * torch.exp(u * mask)).numpy(), (s0 * torch.exp(v)).numpy(), hist

@@ -32,7 +32,7 @@ HIP_FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
              "-munsafe-fp-atomics", "-Wno-unused-result"]
 
 HOST_SOURCES = ["host_formats.cpp", "host_geometry.cpp"]
-HIP_SOURCES = ["fvm_context.hip", "fvm_batch.hip"]
+HIP_SOURCES = ["fvm_context.hip", "fvm_batch.hip", "fvm_adjoint.hip"]
 
 
 def _run(cmd: list[str]) -> None:

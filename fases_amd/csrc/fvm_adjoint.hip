@@ -1,0 +1,432 @@
+// Reverse-time adjoint of the FVM run: gradients of an IR misfit J with respect to the
+// ADE material parameters (R_inf, residues A, B, C, pole real parts zeta, alpha and the
+// pair frequencies beta) and to the source drive (amplitude of each source and the
+// shared waveform samples).
+//
+// Memory: the forward run stores a checkpoint (U^n, the ADE state, the FP64 patch
+// shadows) every `every` steps. The backward sweep walks the segments last to first:
+// it restores the segment's checkpoint, recomputes the segment forward into per-step
+// slots (the inputs V0, V1, V2 of each step's three stages) and then runs the adjoint
+// steps of the segment in reverse, each stage in reverse order: boundary + ADE adjoint,
+// source gradients, flux adjoint, probe-adjoint gather.
+//
+// Scope: FP32 or FP64 field storage, no cPML (the adjoint of the PML states is not part
+// of this library; fases_adjoint_forward rejects a context with PML faces).
+
+#include "fvm_adjoint.hpp"
+#include "fvm_ctx.hpp"
+
+#include <cstring>
+#include <numeric>
+
+namespace {
+
+// Device buffers of the adjoint, owned by the context (freed with it).
+struct AdjBufs {
+    std::vector<void *> allocs;
+    int every = 0, n_ckpt = 0;
+    long long nsteps = 0, step0 = 0;
+    size_t esz = 0;
+    int P = 0;                                   // gradient entries per material
+    // sizes the buffers were made for (reused by a later forward of the same shape)
+    long long key[8] = {};
+    // checkpoints: fields (4 arrays), patch shadows [4 np], ADE state
+    std::vector<void *> ck_f[4];
+    std::vector<double *> ck_sh;
+    std::vector<AdeState> ck_x;
+    // segment slots: V0[0..every], V1[0..every-1], V2[0..every-1] (+ shadows), ADE x0 per step
+    std::vector<void *> v0[4], v1[4], v2[4];
+    std::vector<double *> s0, s1, s2;
+    std::vector<AdeState> xs;
+    // adjoint fields (R precision): next (U^{n+1}), V2, V1, U0
+    void *lam[4][4] = {};
+    AdeState lx[4] = {}, rx1 = {}, rx2 = {};
+    double *lD = nullptr, *G = nullptr, *gs = nullptr, *gus = nullptr, *dj = nullptr;
+    long long gus_n = 0;
+    // probe gather CSR
+    int nT = 0;
+    int *tgt = nullptr, *toff = nullptr, *eb = nullptr;
+    double *ew = nullptr;
+    ~AdjBufs() {
+        for (void *p : allocs) (void)hipFree(p);
+    }
+};
+
+void release_adj(void *p) { delete static_cast<AdjBufs *>(p); }
+
+template <typename T> int adj_alloc(fases_ctx *ctx, AdjBufs &A, T **out, size_t count) {
+    *out = nullptr;
+    void *p = nullptr;
+    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        ctx->err = "adjoint: hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e);
+        return FASES_E_NOMEM;
+    }
+    A.allocs.push_back(p);
+    e = hipMemsetAsync(p, 0, bytes, ctx->stream);
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return FASES_E_HIP; }
+    *out = static_cast<T *>(p);
+    return FASES_OK;
+}
+
+int alloc_ade(fases_ctx *ctx, AdjBufs &A, AdeState &x) {
+    int rc;
+    if ((rc = adj_alloc(ctx, A, &x.phi, (size_t)ctx->nB * ctx->K))) return rc;
+    if ((rc = adj_alloc(ctx, A, &x.p1, (size_t)ctx->nB * ctx->L))) return rc;
+    return adj_alloc(ctx, A, &x.p2, (size_t)ctx->nB * ctx->L);
+}
+
+int copy_ade(fases_ctx *ctx, const AdeState &dst, const double *phi, const double *p1, const double *p2) {
+    const size_t nk = (size_t)ctx->nB * ctx->K * 8, nl = (size_t)ctx->nB * ctx->L * 8;
+    HIPCHK(hipMemcpyAsync(dst.phi, phi, nk, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dst.p1, p1, nl, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dst.p2, p2, nl, hipMemcpyDeviceToDevice, ctx->stream));
+    return FASES_OK;
+}
+
+int zero_ade(fases_ctx *ctx, const AdeState &x) {
+    HIPCHK(hipMemsetAsync(x.phi, 0, (size_t)ctx->nB * ctx->K * 8, ctx->stream));
+    HIPCHK(hipMemsetAsync(x.p1, 0, (size_t)ctx->nB * ctx->L * 8, ctx->stream));
+    HIPCHK(hipMemsetAsync(x.p2, 0, (size_t)ctx->nB * ctx->L * 8, ctx->stream));
+    return FASES_OK;
+}
+
+// Probe-adjoint CSR: for every (boundary cell, trilinear corner) the target grid cell
+// and the corner weight wx wy wz, grouped by target.
+int build_probe_csr(fases_ctx *ctx, AdjBufs &A) {
+    const int nB = ctx->nB;
+    std::vector<int> base((size_t)nB), mat((size_t)nB);
+    std::vector<double> geo((size_t)nB * 8);
+    HIPCHK(hipMemcpy(base.data(), ctx->d_bbase, (size_t)nB * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(mat.data(), ctx->d_bmat, (size_t)nB * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(geo.data(), ctx->d_bgeo, (size_t)nB * 64, hipMemcpyDeviceToHost));
+    struct E { int tgt, b; double w; };
+    std::vector<E> es;
+    es.reserve((size_t)nB * 8);
+    const long long nx = ctx->Nx, nxy = (long long)ctx->Nx * ctx->Ny;
+    for (int b = 0; b < nB; ++b) {
+        if (mat[(size_t)b] & 0x100) { es.push_back({base[(size_t)b], b, 1.0}); continue; }
+        const double sx = geo[(size_t)b * 8], sy = geo[(size_t)b * 8 + 1], sz = geo[(size_t)b * 8 + 2];
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const double w = (dx ? sx : 1.0 - sx) * (dy ? sy : 1.0 - sy) * (dz ? sz : 1.0 - sz);
+                    es.push_back({(int)(base[(size_t)b] + dx + nx * dy + nxy * dz), b, w});
+                }
+    }
+    std::stable_sort(es.begin(), es.end(), [](const E &a, const E &b) { return a.tgt < b.tgt; });
+    std::vector<int> tgt, off, eb;
+    std::vector<double> ew;
+    for (size_t q = 0; q < es.size(); ++q) {
+        if (q == 0 || es[q].tgt != es[q - 1].tgt) { tgt.push_back(es[q].tgt); off.push_back((int)q); }
+        eb.push_back(es[q].b);
+        ew.push_back(es[q].w);
+    }
+    off.push_back((int)es.size());
+    A.nT = (int)tgt.size();
+    int rc;
+    if ((rc = adj_alloc(ctx, A, &A.tgt, tgt.size()))) return rc;
+    if ((rc = adj_alloc(ctx, A, &A.toff, off.size()))) return rc;
+    if ((rc = adj_alloc(ctx, A, &A.eb, eb.size()))) return rc;
+    if ((rc = adj_alloc(ctx, A, &A.ew, ew.size()))) return rc;
+    HIPCHK(hipMemcpy(A.tgt, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(A.toff, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(A.eb, eb.data(), eb.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(A.ew, ew.data(), ew.size() * 8, hipMemcpyHostToDevice));
+    return FASES_OK;
+}
+
+template <typename S> FieldSet<S> fs_of(const std::vector<void *> (&v)[4], size_t q) {
+    return FieldSet<S>{(S *)v[0][q], (S *)v[1][q], (S *)v[2][q], (S *)v[3][q]};
+}
+
+template <typename S, typename R>
+int setup(fases_ctx *ctx, AdjBufs &A, long long nsteps, int every) {
+    A.every = every;
+    A.nsteps = nsteps;
+    A.n_ckpt = (int)((nsteps + every - 1) / every);
+    A.esz = sizeof(S);
+    A.P = 1 + 2 * ctx->K + 4 * ctx->L;
+    const size_t N = (size_t)ctx->N, np = (size_t)std::max(ctx->n_patch, 0);
+    int rc;
+    for (int c = 0; c < A.n_ckpt; ++c) {
+        for (int f = 0; f < 4; ++f) {
+            S *p;
+            if ((rc = adj_alloc(ctx, A, &p, N))) return rc;
+            A.ck_f[f].push_back(p);
+        }
+        double *sh = nullptr;
+        if (np && (rc = adj_alloc(ctx, A, &sh, 4 * np))) return rc;
+        A.ck_sh.push_back(sh);
+        AdeState x{};
+        if ((rc = alloc_ade(ctx, A, x))) return rc;
+        A.ck_x.push_back(x);
+    }
+    const int ns = (int)std::min<long long>(every, nsteps);    // slots of the longest segment
+    for (int q = 0; q <= ns; ++q) {
+        for (int f = 0; f < 4; ++f) {
+            S *p;
+            if ((rc = adj_alloc(ctx, A, &p, N))) return rc;
+            A.v0[f].push_back(p);
+            if (q < ns) {
+                if ((rc = adj_alloc(ctx, A, &p, N))) return rc;
+                A.v1[f].push_back(p);
+                if ((rc = adj_alloc(ctx, A, &p, N))) return rc;
+                A.v2[f].push_back(p);
+            }
+        }
+        double *sh = nullptr;
+        if (np && (rc = adj_alloc(ctx, A, &sh, 4 * np))) return rc;
+        A.s0.push_back(sh);
+        if (q < ns) {
+            if (np && (rc = adj_alloc(ctx, A, &sh, 4 * np))) return rc;
+            A.s1.push_back(np ? sh : nullptr);
+            if (np && (rc = adj_alloc(ctx, A, &sh, 4 * np))) return rc;
+            A.s2.push_back(np ? sh : nullptr);
+            AdeState x{};
+            if ((rc = alloc_ade(ctx, A, x))) return rc;
+            A.xs.push_back(x);
+        }
+    }
+    for (int s = 0; s < 4; ++s)
+        for (int f = 0; f < 4; ++f) {
+            R *p;
+            if ((rc = adj_alloc(ctx, A, &p, N))) return rc;
+            A.lam[s][f] = p;
+        }
+    for (auto &x : A.lx)
+        if ((rc = alloc_ade(ctx, A, x))) return rc;
+    if ((rc = alloc_ade(ctx, A, A.rx1)) || (rc = alloc_ade(ctx, A, A.rx2))) return rc;
+    if ((rc = adj_alloc(ctx, A, &A.lD, (size_t)ctx->nB * 4))) return rc;
+    if ((rc = adj_alloc(ctx, A, &A.G, (size_t)ctx->nB * A.P))) return rc;
+    if ((rc = adj_alloc(ctx, A, &A.gs, (size_t)std::max(ctx->n_src, 1)))) return rc;
+    if ((rc = adj_alloc(ctx, A, &A.dj, (size_t)std::max(ctx->n_rec, 1) * nsteps))) return rc;
+    return build_probe_csr(ctx, A);
+}
+
+template <typename S, typename R>
+int forward_impl(fases_ctx *ctx, long long nsteps, int every, double *ir) {
+    const long long key[8] = {nsteps, every, ctx->N, ctx->nB, (long long)ctx->K * 1000 + ctx->L, ctx->n_patch,
+                              (long long)ctx->n_src * 100000 + ctx->n_rec, (long long)sizeof(S)};
+    AdjBufs *A = static_cast<AdjBufs *>(ctx->adj);
+    const bool reuse = A && std::equal(key, key + 8, A->key);
+    int rc;
+    if (!reuse) {
+        ctx->release_adjoint();
+        A = new (std::nothrow) AdjBufs();
+        if (!A) return set_error(ctx, FASES_E_NOMEM, "adjoint: out of host memory");
+        ctx->adj = A;
+        ctx->adj_release = release_adj;
+        if ((rc = setup<S, R>(ctx, *A, nsteps, every))) { ctx->release_adjoint(); return rc; }
+        std::copy(key, key + 8, A->key);
+    }
+    A->step0 = ctx->step;
+    if ((rc = prepare_run(ctx, nsteps))) return rc;
+    if (uses_tiled(ctx) && ctx->tz == 0 && (rc = autotune_tile<S, R>(ctx))) return rc;
+    const size_t N = (size_t)ctx->N, np = (size_t)std::max(ctx->n_patch, 0);
+    for (long long q = 0; q < nsteps; ++q) {
+        if (q % every == 0) {
+            const int c = (int)(q / every);
+            for (int f = 0; f < 4; ++f)
+                HIPCHK(hipMemcpyAsync(A->ck_f[f][(size_t)c], ctx->U[0][f], N * sizeof(S), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+            if (np)
+                HIPCHK(hipMemcpyAsync(A->ck_sh[(size_t)c], ctx->shadow[0], 4 * np * 8, hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+            if ((rc = copy_ade(ctx, A->ck_x[(size_t)c], ctx->phi[1], ctx->psi1[1], ctx->psi2[1]))) return rc;
+        }
+        if ((rc = launch_step<S, R>(ctx, ctx->step, nullptr))) return rc;
+        if ((rc = launch_receivers<S>(ctx, nsteps, q))) return rc;
+        ctx->step += 1;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipGetLastError());
+    const long long need = (long long)ctx->n_rec * nsteps;
+    if (ir && need > 0) {
+        HIPCHK(hipMemcpy(ir, ctx->d_ir, (size_t)need * sizeof(double), hipMemcpyDeviceToHost));
+        for (long long i = 0; i < need; ++i)
+            if (!std::isfinite(ir[i])) return set_error(ctx, FASES_E_NONFINITE, "receiver sample is not finite");
+    }
+    return FASES_OK;
+}
+
+template <typename R> FieldSet<R> lam_set(AdjBufs &A, int s) {
+    return FieldSet<R>{(R *)A.lam[s][0], (R *)A.lam[s][1], (R *)A.lam[s][2], (R *)A.lam[s][3]};
+}
+
+template <typename S, typename R>
+int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *grad_sval, double *grad_us) {
+    AdjBufs &A = *static_cast<AdjBufs *>(ctx->adj);
+    const fases_params &P = ctx->prm;
+    const long long nsteps = A.nsteps;
+    const size_t N = (size_t)ctx->N, np = (size_t)std::max(ctx->n_patch, 0);
+    const Grid g{ctx->Nx, ctx->Ny, ctx->Nz, (long long)ctx->Nx * ctx->Ny};
+    const Scl<R> sc = field_scales<R>(ctx);
+    const Scl<double> scd = field_scales<double>(ctx);
+    const Ade ade{ctx->K, ctx->L, ctx->d_Rinf, ctx->d_A, ctx->d_zeta, ctx->d_B, ctx->d_C, ctx->d_alpha, ctx->d_beta,
+                  ctx->d_Keff, ctx->d_Leff};
+    const Bnd<double> bd{ctx->nB, ctx->d_bcell, ctx->d_bbase, ctx->d_bmat, ctx->d_bgeo};
+    const double As = 6.0 * P.h * P.h;
+    const double coef0 = P.dt * P.c * P.c * P.rho * As / (P.h * P.h * P.h);
+    int rc;
+    // inputs and zeroed accumulators
+    if (ctx->n_rec > 0)
+        HIPCHK(hipMemcpyAsync(A.dj, dJ, (size_t)ctx->n_rec * nsteps * 8, hipMemcpyHostToDevice, ctx->stream));
+    for (int s = 0; s < 4; ++s)
+        for (int f = 0; f < 4; ++f) HIPCHK(hipMemsetAsync(A.lam[s][f], 0, N * sizeof(R), ctx->stream));
+    for (auto &x : A.lx)
+        if ((rc = zero_ade(ctx, x))) return rc;
+    HIPCHK(hipMemsetAsync(A.G, 0, (size_t)ctx->nB * A.P * 8, ctx->stream));
+    HIPCHK(hipMemsetAsync(A.gs, 0, (size_t)std::max(ctx->n_src, 1) * 8, ctx->stream));
+    const long long gus_n = (ctx->n_src > 0) ? ctx->n_samp * (ctx->src_opt == FASES_SRC_SINE ? ctx->n_src : 1) : 0;
+    if (grad_us && gus_n > 0 && ctx->src_opt != FASES_SRC_SINE) {
+        if (A.gus_n < gus_n) {
+            if ((rc = adj_alloc(ctx, A, &A.gus, (size_t)gus_n))) return rc;
+            A.gus_n = gus_n;
+        }
+        HIPCHK(hipMemsetAsync(A.gus, 0, (size_t)gus_n * 8, ctx->stream));
+    }
+    double *gus = (grad_us && ctx->src_opt != FASES_SRC_SINE && gus_n > 0) ? A.gus : nullptr;
+    // logical adjoint sets: index into A.lam
+    int iNext = 0, iV2 = 1, iV1 = 2, iU0 = 3;
+    // ADE adjoints: lx[ix3] = lambda of X^{n+1}, ix0..ix2 zeroed per step
+    int ix0 = 0, ix1 = 1, ix2 = 2, ix3 = 3;
+    const dim3 fb(64, 4, 1), fgrid((ctx->Nx - 6 + 63) / 64, (ctx->Ny - 6 + 3) / 4, ctx->Nz - 6);
+    const int sb = 256;
+    const dim3 bgrid((ctx->nB + sb - 1) / sb), tgrid((A.nT + sb - 1) / sb);
+    const R kp = (R)(0.5 * P.rho * P.c * P.c), kc = (R)(0.5 * P.c), kr = (R)(0.5 / P.rho);
+    for (int c = A.n_ckpt - 1; c >= 0; --c) {
+        const long long n0 = A.step0 + (long long)c * A.every;
+        const long long len = std::min<long long>(A.every, A.step0 + nsteps - n0);
+        // restore the checkpoint into slot 0 and the context's ADE state
+        for (int f = 0; f < 4; ++f)
+            HIPCHK(hipMemcpyAsync(A.v0[f][0], A.ck_f[f][(size_t)c], N * sizeof(S), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        if (np)
+            HIPCHK(hipMemcpyAsync(A.s0[0], A.ck_sh[(size_t)c], 4 * np * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        {
+            const AdeState &x = A.ck_x[(size_t)c];
+            const size_t nk = (size_t)ctx->nB * ctx->K * 8, nl = (size_t)ctx->nB * ctx->L * 8;
+            HIPCHK(hipMemcpyAsync(ctx->phi[1], x.phi, nk, hipMemcpyDeviceToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(ctx->psi1[1], x.p1, nl, hipMemcpyDeviceToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(ctx->psi2[1], x.p2, nl, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        // recompute the segment into the slots
+        for (long long q = 0; q < len; ++q) {
+            if ((rc = copy_ade(ctx, A.xs[(size_t)q], ctx->phi[1], ctx->psi1[1], ctx->psi2[1]))) return rc;
+            const FieldSet<S> sets[4] = {fs_of<S>(A.v0, (size_t)q), fs_of<S>(A.v1, (size_t)q),
+                                         fs_of<S>(A.v2, (size_t)q), fs_of<S>(A.v0, (size_t)q + 1)};
+            double *const sh[4] = {A.s0[(size_t)q], np ? A.s1[(size_t)q] : nullptr, np ? A.s2[(size_t)q] : nullptr,
+                                   A.s0[(size_t)q + 1]};
+            if ((rc = launch_stages<S, R>(ctx, n0 + q, sets, sh, nullptr))) return rc;
+            std::swap(ctx->phi[0], ctx->phi[1]);
+            std::swap(ctx->psi1[0], ctx->psi1[1]);
+            std::swap(ctx->psi2[0], ctx->psi2[1]);
+        }
+        // adjoint steps of the segment, last first
+        for (long long q = len - 1; q >= 0; --q) {
+            const long long n = n0 + q;
+            const FieldSet<S> V0 = fs_of<S>(A.v0, (size_t)q), V1 = fs_of<S>(A.v1, (size_t)q),
+                              V2 = fs_of<S>(A.v2, (size_t)q);
+            const AdeState &X0 = A.xs[(size_t)q];
+            if (ctx->n_rec > 0)
+                hipLaunchKernelGGL((k_receiver_adj<R>), dim3(1), dim3(64), 0, ctx->stream, (R *)A.lam[iNext][0],
+                                   ctx->d_rcell, ctx->n_rec, A.dj, nsteps, n - A.step0);
+            if (ctx->nB > 0)
+                hipLaunchKernelGGL((k_ade_replay<S>), bgrid, dim3(sb), 0, ctx->stream, g, V0, V1, bd, ade, P.rho, P.c,
+                                   P.dt, X0, A.rx1, A.rx2, scd);
+            if ((rc = zero_ade(ctx, A.lx[ix0])) || (rc = zero_ade(ctx, A.lx[ix1])) || (rc = zero_ade(ctx, A.lx[ix2])))
+                return rc;
+            // stage table: (input set, output adjoint, input adjoint, ADE right state, lnew, lr,
+            //               a, b, c, RK alpha / beta, source weight and sample)
+            struct St {
+                FieldSet<S> vin; int lout, lin; const AdeState *xr; int lnew, lr;
+                double a, b, c, alpha, beta, w; long long nn;
+            };
+            const St st[3] = {
+                {V0, iV1, iU0, &X0, ix1, ix0, 0.0, 1.0, 1.0, 0.0, 1.0, 1.0, 2 * n},
+                {V1, iV2, iV1, &A.rx1, ix2, ix1, 3.0, 1.0, 4.0, 0.75, 0.25, 0.25, 2 * n + 2},
+                {V2, iNext, iV2, &A.rx2, ix3, ix2, 1.0, 2.0, 3.0, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 2 * n + 1},
+            };
+            for (int s = 2; s >= 0; --s) {
+                const St &t = st[s];
+                const FieldSet<R> Lo = lam_set<R>(A, t.lout), Li = lam_set<R>(A, t.lin), Lb = lam_set<R>(A, iU0);
+                if (ctx->nB > 0)
+                    hipLaunchKernelGGL((k_boundary_adj<S, R>), bgrid, dim3(sb), 0, ctx->stream, g, t.vin, bd, ade,
+                                       P.rho, P.c, P.dt, t.a, t.b, t.c, X0, *t.xr, A.lx[t.lnew], A.lx[ix0], A.lx[t.lr],
+                                       Lo.p, Lo.u, Lo.v, Lo.w, A.lD, A.G, A.P, scd);
+                if (ctx->n_src > 0)
+                    hipLaunchKernelGGL((k_source_adj<R>), dim3((ctx->n_src + 63) / 64), dim3(64), 0, ctx->stream, Lo.p,
+                                       ctx->d_scell, ctx->d_sval, ctx->d_us,
+                                       ctx->src_opt == FASES_SRC_SINE ? ctx->n_samp : 0LL, ctx->n_src, t.nn,
+                                       t.w * coef0, A.gs, gus);
+                const AdjK<R> K{kp, kc, kr, (R)(t.beta * P.dt / P.h), (R)t.alpha, (R)t.beta};
+                if (s == 2)
+                    hipLaunchKernelGGL((k_flux_adj_cell<S, R, false, false, true>), fgrid, fb, 0, ctx->stream, g,
+                                       t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w, ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p,
+                                       Lb.u, Lb.v, Lb.w, K);
+                else if (s == 1)
+                    hipLaunchKernelGGL((k_flux_adj_cell<S, R, false, true, true>), fgrid, fb, 0, ctx->stream, g,
+                                       t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w, ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p,
+                                       Lb.u, Lb.v, Lb.w, K);
+                else
+                    hipLaunchKernelGGL((k_flux_adj_cell<S, R, true, false, false>), fgrid, fb, 0, ctx->stream, g,
+                                       t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w, ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p,
+                                       Lb.u, Lb.v, Lb.w, K);
+                if (ctx->nB > 0)
+                    hipLaunchKernelGGL((k_probe_gather<R>), tgrid, dim3(sb), 0, ctx->stream, A.nT, A.tgt, A.toff, A.eb,
+                                       A.ew, A.lD, Li.p, Li.u, Li.v, Li.w);
+            }
+            std::swap(iNext, iU0);
+            std::swap(ix3, ix0);
+        }
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipGetLastError());
+    // material gradients: sum the per-cell contributions by material
+    if (grad_mat) {
+        std::vector<double> Gh((size_t)ctx->nB * A.P);
+        std::vector<int> mat((size_t)ctx->nB);
+        HIPCHK(hipMemcpy(Gh.data(), A.G, Gh.size() * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(mat.data(), ctx->d_bmat, mat.size() * 4, hipMemcpyDeviceToHost));
+        std::fill(grad_mat, grad_mat + (size_t)ctx->nMat * A.P, 0.0);
+        for (int b = 0; b < ctx->nB; ++b) {
+            double *o = grad_mat + (size_t)(mat[(size_t)b] & 0xFF) * A.P;
+            for (int p = 0; p < A.P; ++p) o[p] += Gh[(size_t)b * A.P + p];
+        }
+    }
+    if (grad_sval && ctx->n_src > 0) HIPCHK(hipMemcpy(grad_sval, A.gs, (size_t)ctx->n_src * 8, hipMemcpyDeviceToHost));
+    if (grad_us && gus) HIPCHK(hipMemcpy(grad_us, gus, (size_t)gus_n * 8, hipMemcpyDeviceToHost));
+    return FASES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fases_adjoint_forward(fases_ctx *ctx, int64_t nsteps, int checkpoint_every, double *ir) {
+    if (!ctx) return FASES_E_ARG;
+    if (!ctx->have_geo || !ctx->have_mat)
+        return set_error(ctx, FASES_E_STATE, "fases_adjoint_forward: geometry and materials must be set first");
+    if (nsteps < 1 || checkpoint_every < 1)
+        return set_error(ctx, FASES_E_ARG, "fases_adjoint_forward: nsteps and checkpoint_every must be >= 1");
+    if (ctx->pml.on)
+        return set_error(ctx, FASES_E_ARG, "fases_adjoint_forward: the adjoint has no cPML; disable the PML faces");
+    if (ctx->prm.precision == FASES_FP16)
+        return set_error(ctx, FASES_E_ARG, "fases_adjoint_forward: FP16 storage has no adjoint; use FP32 or FP64");
+    HIPCHK(hipSetDevice(ctx->prm.device));
+    if (ctx->prm.precision == FASES_FP64) return forward_impl<double, double>(ctx, nsteps, checkpoint_every, ir);
+    return forward_impl<float, float>(ctx, nsteps, checkpoint_every, ir);
+}
+
+int fases_adjoint(fases_ctx *ctx, const double *dJ_dir, double *grad_mat, double *grad_sval, double *grad_us) {
+    if (!ctx) return FASES_E_ARG;
+    if (!ctx->adj) return set_error(ctx, FASES_E_STATE, "fases_adjoint: run fases_adjoint_forward first");
+    if (ctx->n_rec > 0 && !dJ_dir) return set_error(ctx, FASES_E_ARG, "fases_adjoint: dJ_dir is NULL");
+    HIPCHK(hipSetDevice(ctx->prm.device));
+    if (ctx->prm.precision == FASES_FP64) return backward_impl<double, double>(ctx, dJ_dir, grad_mat, grad_sval, grad_us);
+    return backward_impl<float, float>(ctx, dJ_dir, grad_mat, grad_sval, grad_us);
+}
+
+}  // extern "C"

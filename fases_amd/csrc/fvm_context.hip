@@ -102,104 +102,6 @@ int exponent_for(double amax, int keep) {
     return e - kHalfTop;
 }
 
-template <typename S, typename R>
-int launch_step(fases_ctx *ctx, long long n, std::vector<hipEvent_t> *kev) {
-    int rc = FASES_OK;
-    Grid g{ctx->Nx, ctx->Ny, ctx->Nz, (long long)ctx->Nx * ctx->Ny};
-    const fases_params &P = ctx->prm;
-    const R rho = (R)P.rho, c = (R)P.c, h = (R)P.h, dt = (R)P.dt;
-    const Scl<R> sc = field_scales<R>(ctx);
-    const Scl<double> scd = field_scales<double>(ctx);
-    dim3 blk(64, 4, 1);
-    dim3 grd((ctx->Nx - 2 * NG + 63) / 64, (ctx->Ny - 2 * NG + 3) / 4, ctx->Nz - 2 * NG);
-    const double As = 6.0 * P.h * P.h;
-    const int sb = 256;
-    Ade ade{ctx->K, ctx->L, ctx->d_Rinf, ctx->d_A, ctx->d_zeta, ctx->d_B, ctx->d_C, ctx->d_alpha, ctx->d_beta,
-            ctx->d_Keff, ctx->d_Leff};
-    // Geometry stays FP64 for every field precision: a blend weight or normal rounded
-    // once to FP32 is a bias that every step repeats (measured: FP32-room IR error x3.5).
-    typedef double G;
-    Bnd<G> bd{ctx->nB, ctx->d_bcell, ctx->d_bbase, ctx->d_bmat, ctx->d_bgeo};
-    // (field set in, set out, RK weight, source sample, ADE a,b,c, ADE buffers left/mid/right)
-    struct StageCfg { int in, out; double w; long long nn; double a, b, c; int l, mid, r; };
-    const StageCfg cfg[3] = {
-        {0, 1, 1.0, 2 * n, 0.0, 1.0, 1.0, 0, 1, 1},
-        {1, 2, 0.25, 2 * n + 2, 3.0, 1.0, 4.0, 2, 1, 0},
-        {2, 1, 2.0 / 3.0, 2 * n + 1, 1.0, 2.0, 3.0, 0, 1, 2},
-    };
-    for (int s = 0; s < 3; ++s) {
-        const StageCfg &q = cfg[s];
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (kev) {
-            HIPCHK(hipEventCreate(&e0));
-            HIPCHK(hipEventCreate(&e1));
-            kev->push_back(e0);
-            kev->push_back(e1);
-            HIPCHK(hipEventRecord(e0, ctx->stream));
-        }
-        if (!uses_tiled(ctx)) {
-            const GridIO<S, R> gio{fset<S>(ctx, q.in), fset<S>(ctx, 0), fset<S>(ctx, q.out), sc};
-            if (s == 0)
-                hipLaunchKernelGGL((k_flux_simple<S, R, 0>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml,
-                                   rho, c, h, dt);
-            else if (s == 1)
-                hipLaunchKernelGGL((k_flux_simple<S, R, 1>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml,
-                                   rho, c, h, dt);
-            else
-                hipLaunchKernelGGL((k_flux_simple<S, R, 2>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml,
-                                   rho, c, h, dt);
-        } else {
-            rc = launch_tiled<S, R, 16>(ctx, s, q.in, q.out, rho, c, h, dt, ctx->tz);
-            if (rc) return rc;
-        }
-        if (kev) HIPCHK(hipEventRecord(e1, ctx->stream));
-        if (ctx->n_patch > 0) {
-            const dim3 pg((ctx->n_patch + 63) / 64), pb(64);
-            const PatchIO<S> pio{fset<S>(ctx, q.in), fset<S>(ctx, 0), fset<S>(ctx, q.out), ctx->d_pidx,
-                                 ctx->shadow[q.in], ctx->shadow[0], ctx->shadow[q.out], ctx->n_patch, scd};
-            if (s == 0)
-                hipLaunchKernelGGL((k_flux_patch<S, 0>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.rho, P.c, P.h, P.dt);
-            else if (s == 1)
-                hipLaunchKernelGGL((k_flux_patch<S, 1>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.rho, P.c, P.h, P.dt);
-            else
-                hipLaunchKernelGGL((k_flux_patch<S, 2>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.rho, P.c, P.h, P.dt);
-        }
-        if (ctx->n_src > 0)
-            hipLaunchKernelGGL((k_sources<S, R>), dim3((ctx->n_src + sb - 1) / sb), dim3(sb), 0, ctx->stream,
-                               (S *)ctx->U[q.out][0], ctx->d_scell, ctx->d_sval, ctx->d_sfreq, ctx->d_us,
-                               ctx->src_opt == FASES_SRC_SINE ? ctx->n_samp : 0LL, ctx->n_src, ctx->src_opt, q.nn, q.w, P.dt, P.c, P.rho, As, P.h,
-                               ctx->n_patch > 0 ? ctx->d_pidx : nullptr, ctx->shadow[q.out], sc.s[0], sc.is[0]);
-        if (ctx->nB > 0)
-            hipLaunchKernelGGL((k_boundary<S, G>), dim3((ctx->nB + sb - 1) / sb), dim3(sb), 0, ctx->stream, g,
-                               fset<S>(ctx, q.in), fset<S>(ctx, q.out), bd, ade, P.rho, P.c, P.dt, q.a, q.b, q.c,
-                               AdeBufs{ctx->phi[q.l], ctx->phi[q.mid], ctx->phi[q.r], ctx->psi1[q.l],
-                                       ctx->psi1[q.mid], ctx->psi1[q.r], ctx->psi2[q.l], ctx->psi2[q.mid],
-                                       ctx->psi2[q.r]},
-                               scd);
-    }
-    // rotate: U0 <-> U1, phi_old <-> phi (main3D.cu:1312-1319)
-    for (int f = 0; f < 4; ++f) std::swap(ctx->U[0][f], ctx->U[1][f]);
-    std::swap(ctx->phi[0], ctx->phi[1]);
-    std::swap(ctx->psi1[0], ctx->psi1[1]);
-    std::swap(ctx->psi2[0], ctx->psi2[1]);
-    std::swap(ctx->shadow[0], ctx->shadow[1]);
-    return FASES_OK;
-}
-
-template <typename S>
-int launch_receivers(fases_ctx *ctx, long long stride, long long n) {
-    if (ctx->n_rec <= 0) return FASES_OK;
-    hipLaunchKernelGGL((k_receivers<S>), dim3((ctx->n_rec + 63) / 64), dim3(64), 0, ctx->stream, fset<S>(ctx, 0),
-                       ctx->d_rcell, ctx->n_rec, ctx->d_ir, ctx->d_vel, stride, n, field_scales<double>(ctx));
-    return FASES_OK;
-}
-
-// FP16 rescale pass: measure max |value| per field on the current set, pick new
-// exponents, multiply the stored values of all three sets by 2^(old - new). Fails with
-// FASES_E_NONFINITE when a stored value has overflowed.
 template <typename S>
 int rescale_fields(fases_ctx *ctx) {
     if (!ctx->d_amax) {
@@ -344,6 +246,7 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
     if (N >= (1LL << 31)) return set_error(ctx, FASES_E_ARG, "fases_set_geometry: grid exceeds 2^31 cells");
     HIPCHK(hipSetDevice(ctx->prm.device));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->release_adjoint();
     dfree_all(ctx);
     ctx->have_geo = ctx->have_mat = false;
     ctx->Nx = Nx; ctx->Ny = Ny; ctx->Nz = Nz; ctx->N = N;

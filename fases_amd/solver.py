@@ -309,6 +309,7 @@ class Solver:
     def set_materials(self, m: Materials) -> None:
         arrs = [_f64(a) for a in (m.Rinf, m.A, m.zeta, m.B, m.C, m.alpha, m.beta)]
         self._chk(self._lib.fases_set_materials(self._ctx, m.n, m.K, m.L, *[_ptr(a, C.c_double) for a in arrs]))
+        self.mat_shape = (m.n, m.K, m.L)
 
     def set_sources(self, cells, sval, src_opt: int = _lib.FASES_SRC_RICKER, u_s=None, freq=None) -> None:
         cells = np.ascontiguousarray(cells, np.int64)
@@ -318,6 +319,8 @@ class Solver:
         self._chk(self._lib.fases_set_sources(self._ctx, cells.size, _ptr(cells, C.c_int64), _ptr(sval, C.c_double),
                                               src_opt, _ptr(u, C.c_double), 0 if u is None else u.size,
                                               _ptr(fr, C.c_double)))
+        self.n_src = len(set(int(c) for c in cells)) if src_opt != _lib.FASES_SRC_SINE else cells.size
+        self.n_samp = 0 if u is None else u.size
 
     def set_receivers(self, cells) -> None:
         cells = np.ascontiguousarray(cells, np.int64)
@@ -351,6 +354,27 @@ class Solver:
         a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
         self._chk(self._lib.fases_info(self._ctx, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         return {"n_cells": a.value, "n_bnd": b.value, "n_upd": c.value, "dev_bytes": d.value}
+
+    def adjoint_forward(self, nsteps: int, checkpoint_every: int = 64) -> np.ndarray:
+        """Forward run storing checkpoints for adjoint(); returns ir [n_rec, nsteps]."""
+        ir = np.zeros((self.n_rec, nsteps))
+        self._chk(self._lib.fases_adjoint_forward(self._ctx, nsteps, checkpoint_every, _ptr(ir, C.c_double)))
+        self._adj_steps = nsteps
+        return ir
+
+    def adjoint(self, dj_dir: np.ndarray, want_us: bool = False):
+        """Reverse sweep for dJ/d ir; returns (grad_mat [nMat, 1+2K+4L], grad_sval [n_src],
+        grad_us [n_samp] or None). Gradient layout per material: R_inf, A[K], zeta[K], B[L],
+        C[L], alpha[L], beta[L]. The field state is undefined afterwards."""
+        steps = getattr(self, "_adj_steps", None)     # None: the library reports the missing forward
+        dj = _f64(dj_dir).reshape(self.n_rec, -1 if steps is None else steps)
+        n, K, L = self.mat_shape
+        gm = np.zeros((n, 1 + 2 * K + 4 * L))
+        gs = np.zeros(max(getattr(self, "n_src", 0), 1))
+        gu = np.zeros(self.n_samp) if (want_us and getattr(self, "n_samp", 0)) else None
+        self._chk(self._lib.fases_adjoint(self._ctx, _ptr(dj, C.c_double), _ptr(gm, C.c_double), _ptr(gs, C.c_double),
+                                          _ptr(gu, C.c_double)))
+        return gm, gs[: getattr(self, "n_src", 0)], gu
 
     def tile_config(self) -> tuple:
         """(columns, rows, z planes) per block of the tiled flux kernel (z = 0 before tuning)."""

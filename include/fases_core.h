@@ -194,6 +194,24 @@ int fases_reset(fases_ctx *ctx);
 int fases_run(fases_ctx *ctx, int64_t nsteps, double *ir, double *vel,
               double *elapsed_ms, double *kernel_ms);
 
+/* Forward run of nsteps steps from the current state that also stores a checkpoint
+ * (fields, ADE states, FP64 patch shadows) every checkpoint_every steps for
+ * fases_adjoint. ir as in fases_run. FP32 or FP64 storage; contexts with PML faces
+ * are rejected (no cPML adjoint). Checkpoint and segment buffers stay allocated in the
+ * context and are reused by the next call of the same shape.
+ * No counterpart in the reference (gradients there come from fitting loops outside the
+ * solver, fminconFIT.py / vectFIT.py driving repeated runs). */
+int fases_adjoint_forward(fases_ctx *ctx, int64_t nsteps, int checkpoint_every, double *ir);
+
+/* Reverse sweep after fases_adjoint_forward: given dJ/d ir [n_rec * nsteps] of a scalar
+ * misfit J of the receiver pressures, writes (each nullable)
+ *   grad_mat  [nMat * (1 + 2K + 4L)]: per material dJ/d(R_inf, A[K], zeta[K], B[L], C[L],
+ *             alpha[L], beta[L]) (poles a material does not use get 0),
+ *   grad_sval [n_src]: dJ/d source drive amplitude,
+ *   grad_us   [n_samp]: dJ/d shared waveform sample (Ricker / PCS sources only).
+ * The context's field state is undefined afterwards: reset or set fields before running. */
+int fases_adjoint(fases_ctx *ctx, const double *dJ_dir, double *grad_mat, double *grad_sval, double *grad_us);
+
 /* Advances n independent contexts (jobs) together: nsteps[j] steps for job j, each
  * stage of all jobs in one launch per kernel kind, the step loop captured into
  * hipGraphs and replayed; jobs drop out as their step counts run out. ir[j]

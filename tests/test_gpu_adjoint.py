@@ -1,0 +1,212 @@
+"""Hand-written adjoint (fases_adjoint_forward / fases_adjoint) against central finite
+differences of the FP64 forward (bit-identical to the FP64 oracle), a dot-product test,
+checkpoint-interval invariance, FP32 vs FP64 gradients, the torch.autograd.Function
+wrapper and a short L-BFGS recovery.
+
+Finite-difference step: WENO-Z weights are very curved where the field is at the level
+of eps (the wave front's tail), so central differences with a relative step of 1e-4 carry
+errors up to ~1e-3 there; the FD reference here uses 1e-6 (measured to converge), and the
+bar is 1e-4 relative."""
+import copy
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from fases_amd import _lib
+from fases_amd import workloads as W
+from fases_amd._lib import FasesError
+
+pytestmark = pytest.mark.gpu
+
+H_FD = 1e-6
+FIELDS = [("Rinf", None), ("A", 0), ("A", 1), ("zeta", 0), ("zeta", 1), ("B", 0), ("C", 0), ("alpha", 0), ("beta", 0)]
+
+
+def announce(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def room(nsteps=150):
+    return W.ade_room(room=(30, 26, 22), h=0.02, nsteps=nsteps, seed=21, n_rec=3, n_types=3, K=2, L=1, pml_x_hi=False)
+
+
+def perturbed(m, seed=5, rel=0.2):
+    rng = np.random.default_rng(seed)
+    m2 = copy.deepcopy(m)
+    for a in (m2.Rinf, m2.A, m2.zeta, m2.B, m2.C, m2.alpha):
+        a *= np.exp(rng.uniform(-rel, rel, a.shape))
+    return m2
+
+
+class Setup:
+    def __init__(self, prec="fp64", nsteps=150):
+        self.wl = room(nsteps)
+        self.sv = W.make_solver(self.wl, prec)
+        self.sv.set_materials(perturbed(self.wl.materials))
+        self.sv.set_fields(self.wl.initial_pressure())
+        self.d, _, _, _ = self.sv.run(self.wl.nsteps)
+        self.sv.set_materials(self.wl.materials)
+
+    def J(self, m=None, sval=None, u_s=None):
+        wl = self.wl
+        self.sv.set_materials(wl.materials if m is None else m)
+        self.sv.set_sources([wl.cell(s) for s in wl.src_ijk], wl.src_sval if sval is None else sval, wl.src_opt,
+                            wl.u_s if u_s is None else u_s)
+        self.sv.set_fields(wl.initial_pressure())
+        r, _, _, _ = self.sv.run(wl.nsteps)
+        return float(((r - self.d) ** 2).sum()), r
+
+    def grad(self, every=64, dj=None):
+        wl = self.wl
+        self.sv.set_materials(wl.materials)
+        self.sv.set_sources([wl.cell(s) for s in wl.src_ijk], wl.src_sval, wl.src_opt, wl.u_s)
+        self.sv.set_fields(wl.initial_pressure())
+        ir = self.sv.adjoint_forward(wl.nsteps, every)
+        return self.sv.adjoint(2 * (ir - self.d) if dj is None else dj, want_us=True), ir
+
+
+@pytest.fixture(scope="module")
+def s64():
+    s = Setup("fp64")
+    yield s
+    s.sv.close()
+
+
+def test_material_gradients_match_central_fd(s64):
+    announce("adjoint vs FD, 3 materials x 9 parameters")
+    (gm, gs, _), _ = s64.grad()
+    fd = np.zeros_like(gm)
+    for mi in range(3):
+        for pi, (name, k) in enumerate(FIELDS):
+            base = getattr(s64.wl.materials, name)
+            v0 = base[mi] if k is None else base[mi, k]
+            h = H_FD * abs(v0)
+            vals = []
+            for sg in (1, -1):
+                m2 = copy.deepcopy(s64.wl.materials)
+                arr = getattr(m2, name)
+                if k is None:
+                    arr[mi] = v0 + sg * h
+                else:
+                    arr[mi, k] = v0 + sg * h
+                vals.append(s64.J(m=m2)[0])
+            fd[mi, pi] = (vals[0] - vals[1]) / (2 * h)
+    gm, fd = gm[:3], fd[:3]
+    err = np.abs(gm - fd)
+    print("max rel err %.2e" % (err / np.abs(fd)).max())
+    assert np.all(err <= 1e-4 * np.abs(fd) + 1e-7 * np.abs(fd).max())
+
+
+def test_source_gradients_match_fd(s64):
+    announce("adjoint vs FD, source amplitude and waveform samples")
+    (gm, gs, gu), _ = s64.grad()
+    s0 = s64.wl.src_sval[0]
+    h = H_FD * s0
+    fd = (s64.J(sval=[s0 + h])[0] - s64.J(sval=[s0 - h])[0]) / (2 * h)
+    assert abs(gs[0] - fd) <= 1e-6 * abs(fd)
+    # waveform samples (the WENO weights make the IR nonlinear in them, so a small step)
+    us = s64.wl.u_s
+    k = int(np.argmax(np.abs(gu)))
+    for idx in (k, k // 2, 2 * k):
+        hs = H_FD * abs(us).max()
+        up, dn = us.copy(), us.copy()
+        up[idx] += hs
+        dn[idx] -= hs
+        fd = (s64.J(u_s=up)[0] - s64.J(u_s=dn)[0]) / (2 * hs)
+        print("u_s[%d]: fd %.8e adjoint %.8e" % (idx, fd, gu[idx]))
+        assert abs(gu[idx] - fd) <= 1e-4 * abs(fd) + 1e-6 * abs(gu).max(), idx
+
+
+def test_dot_product(s64):
+    """<J v, w> = <v, J^T w> with J = d ir / d theta, v a random parameter direction, w a
+    random IR weight: the left side from a directional central difference, the right side
+    from one adjoint sweep with dJ/d ir = w."""
+    announce("dot-product test")
+    rng = np.random.default_rng(3)
+    wl = s64.wl
+    w = rng.standard_normal(s64.d.shape)
+    (gm, gs, _), _ = s64.grad(dj=w)
+    m = wl.materials
+    dirs = {name: rng.standard_normal(getattr(m, name).shape) * np.abs(getattr(m, name)) for name in
+            ("Rinf", "A", "zeta", "B", "C", "alpha")}
+    dsv = rng.standard_normal() * wl.src_sval[0]
+
+    def shifted(t):
+        m2 = copy.deepcopy(m)
+        for name, dv in dirs.items():
+            getattr(m2, name)[...] += t * dv
+        return m2
+
+    lhs_h = []
+    for t in (H_FD, 0.1 * H_FD):
+        rp = s64.J(m=shifted(t), sval=[wl.src_sval[0] + t * dsv])[1]
+        rm = s64.J(m=shifted(-t), sval=[wl.src_sval[0] - t * dsv])[1]
+        lhs_h.append(float(((rp - rm) / (2 * t) * w).sum()))
+    K, L = m.K, m.L
+    rhs = float((gm[:, 0] * dirs["Rinf"]).sum() + (gm[:, 1:1 + K] * dirs["A"]).sum()
+                + (gm[:, 1 + K:1 + 2 * K] * dirs["zeta"]).sum() + (gm[:, 1 + 2 * K:1 + 2 * K + L] * dirs["B"]).sum()
+                + (gm[:, 1 + 2 * K + L:1 + 2 * K + 2 * L] * dirs["C"]).sum()
+                + (gm[:, 1 + 2 * K + 2 * L:1 + 2 * K + 3 * L] * dirs["alpha"]).sum() + gs[0] * dsv)
+    print("<Jv,w> = %.10e (step 1e-6), %.10e (step 1e-7); <v,JTw> = %.10e" % (lhs_h[0], lhs_h[1], rhs))
+    # the difference quotient itself moves by ~1e-4 between steps (curved WENO weights)
+    assert min(abs(v - rhs) for v in lhs_h) <= 1e-4 * abs(rhs)
+
+
+def test_checkpoint_interval_does_not_change_gradients(s64):
+    announce("checkpoint interval 16 vs 64 vs 150")
+    (a, sa, _), _ = s64.grad(every=64)
+    for ev in (16, 150):
+        (b, sb, _), _ = s64.grad(every=ev)
+        assert np.array_equal(a, b) and np.array_equal(sa, sb)
+
+
+def test_fp32_gradients_close_to_fp64(s64):
+    announce("fp32 adjoint vs fp64 adjoint")
+    (g64, s64g, _), _ = s64.grad()
+    s32 = Setup("fp32")
+    (g32, s32g, _), _ = s32.grad()
+    s32.sv.close()
+    for q in range(g64.shape[0]):
+        assert np.abs(g32[q] - g64[q]).max() <= 1e-3 * np.abs(g64[q]).max()
+    assert abs(s32g[0] - s64g[0]) <= 1e-4 * abs(s64g[0])
+
+
+def test_autograd_function_and_lbfgs():
+    torch = pytest.importorskip("torch")
+    from fases_amd.autograd import RoomIR, RoomProblem, lbfgs_recover, pack
+    announce("torch.autograd.Function + L-BFGS")
+    s = Setup("fp64", nsteps=120)
+    prob = RoomProblem(s.sv, s.wl, checkpoint_every=32)
+    th = torch.tensor(pack(s.wl.materials), requires_grad=True)
+    sv_t = torch.tensor(np.asarray(s.wl.src_sval, float), requires_grad=True)
+    ir = RoomIR.apply(th, sv_t, prob)
+    J = ((ir - torch.from_numpy(s.d)) ** 2).sum()
+    J.backward()
+    (gm, gs, _), _ = s.grad()
+    assert np.allclose(th.grad.numpy(), gm, rtol=1e-12, atol=0)
+    assert np.allclose(sv_t.grad.numpy(), gs, rtol=1e-12, atol=0)
+    # a few L-BFGS iterations from the perturbed start towards the target's materials
+    target = s.d
+    theta, sval, hist = lbfgs_recover(prob, target, pack(s.wl.materials), np.asarray(s.wl.src_sval), iterations=8)
+    print("L-BFGS J:", ["%.3e" % v for v in hist])
+    assert hist[-1] < 0.3 * hist[0]
+    s.sv.close()
+
+
+def test_adjoint_rejects_pml_and_fp16():
+    announce("adjoint error paths")
+    wl = W.ade_room(room=(24, 20, 18), h=0.02, nsteps=10, seed=2, n_rec=1, K=1, L=1, pml_cells=4)
+    sv = W.make_solver(wl, "fp64")
+    with pytest.raises(FasesError, match="PML"):
+        sv.adjoint_forward(10, 4)
+    sv.close()
+    wl2 = dataclasses.replace(wl, params=dataclasses.replace(wl.params, pml_keys=(0,) * 6))
+    sv = W.make_solver(wl2, "fp16")
+    with pytest.raises(FasesError, match="FP16"):
+        sv.adjoint_forward(10, 4)
+    with pytest.raises(FasesError, match="adjoint_forward"):
+        sv.adjoint(np.zeros((1, 10)))
+    sv.close()
