@@ -44,6 +44,7 @@ class fases_params(C.Structure):
         ("use_graph", C.c_int),
         ("kernel_variant", C.c_int),
         ("tile_z", C.c_int),
+        ("adjoint_variant", C.c_int),
     ]
 
 

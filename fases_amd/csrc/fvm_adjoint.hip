@@ -21,6 +21,11 @@
 
 namespace {
 
+// Chebyshev radius of the FP64 adjoint around point sources (reduced-precision storage),
+// the forward's FP64 radius: measured on a 60x50x44 room, the FP32 drive gradient is off
+// by a factor 3 with no FP64 adjoint, by 6.5e-4 with radius 6 and by 1e-6 with 12.
+constexpr int kAdjointPatch = 12;
+
 // Device buffers of the adjoint, owned by the context (freed with it).
 struct AdjBufs {
     std::vector<void *> allocs;
@@ -38,8 +43,13 @@ struct AdjBufs {
     std::vector<void *> v0[4], v1[4], v2[4];
     std::vector<double *> s0, s1, s2;
     std::vector<AdeState> xs;
-    // adjoint fields (R precision): next (U^{n+1}), V2, V1, U0
+    // adjoint fields (R precision): next (U^{n+1}), V2, V1, U0, and their FP64 shadows on
+    // the source-patch cells ([4][np] each; reduced-precision storage only)
     void *lam[4][4] = {};
+    double *lam_sh[4] = {};
+    // adjoint patch: the forward patch cells within kAdjointPatch of a source
+    int n_ap = 0;
+    int *ap_cells = nullptr, *ap_idx = nullptr;
     AdeState lx[4] = {}, rx1 = {}, rx2 = {};
     double *lD = nullptr, *G = nullptr, *gs = nullptr, *gus = nullptr, *dj = nullptr;
     long long gus_n = 0;
@@ -195,6 +205,31 @@ int setup(fases_ctx *ctx, AdjBufs &A, long long nsteps, int every) {
             if ((rc = adj_alloc(ctx, A, &p, N))) return rc;
             A.lam[s][f] = p;
         }
+    if (np) {
+        // adjoint patch cells: forward patch cells near a source, in forward patch order
+        std::vector<int> scell((size_t)std::max(ctx->n_src, 0));
+        if (!scell.empty())
+            HIPCHK(hipMemcpy(scell.data(), ctx->d_scell, scell.size() * 4, hipMemcpyDeviceToHost));
+        const long long nx = ctx->Nx, nxy = (long long)ctx->Nx * ctx->Ny;
+        std::vector<int> apc, api((size_t)ctx->N, -1);
+        for (int c : ctx->h_patch) {
+            const long long i = c % nx, j = (c / nx) % ctx->Ny, k = c / nxy;
+            bool near = false;
+            for (int s0 : scell) {
+                const long long si = s0 % nx, sj = (s0 / nx) % ctx->Ny, sk = s0 / nxy;
+                if (std::llabs(i - si) <= kAdjointPatch && std::llabs(j - sj) <= kAdjointPatch &&
+                    std::llabs(k - sk) <= kAdjointPatch) { near = true; break; }
+            }
+            if (near) { api[(size_t)c] = (int)apc.size(); apc.push_back(c); }
+        }
+        A.n_ap = (int)apc.size();
+        if ((rc = adj_alloc(ctx, A, &A.ap_cells, apc.size()))) return rc;
+        if ((rc = adj_alloc(ctx, A, &A.ap_idx, api.size()))) return rc;
+        HIPCHK(hipMemcpy(A.ap_cells, apc.data(), apc.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(A.ap_idx, api.data(), api.size() * 4, hipMemcpyHostToDevice));
+        for (auto &l : A.lam_sh)
+            if ((rc = adj_alloc(ctx, A, &l, 4 * (size_t)std::max(A.n_ap, 1)))) return rc;
+    }
     for (auto &x : A.lx)
         if ((rc = alloc_ade(ctx, A, x))) return rc;
     if ((rc = alloc_ade(ctx, A, A.rx1)) || (rc = alloc_ade(ctx, A, A.rx2))) return rc;
@@ -273,8 +308,12 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
     // inputs and zeroed accumulators
     if (ctx->n_rec > 0)
         HIPCHK(hipMemcpyAsync(A.dj, dJ, (size_t)ctx->n_rec * nsteps * 8, hipMemcpyHostToDevice, ctx->stream));
-    for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < 4; ++s) {
         for (int f = 0; f < 4; ++f) HIPCHK(hipMemsetAsync(A.lam[s][f], 0, N * sizeof(R), ctx->stream));
+        if (A.n_ap) HIPCHK(hipMemsetAsync(A.lam_sh[s], 0, 4 * (size_t)A.n_ap * 8, ctx->stream));
+    }
+    const int nap = A.n_ap;
+    const int *pidx = nap ? A.ap_idx : nullptr;       // cells with an FP64 adjoint
     for (auto &x : A.lx)
         if ((rc = zero_ade(ctx, x))) return rc;
     HIPCHK(hipMemsetAsync(A.G, 0, (size_t)ctx->nB * A.P * 8, ctx->stream));
@@ -296,6 +335,10 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
     const int sb = 256;
     const dim3 bgrid((ctx->nB + sb - 1) / sb), tgrid((A.nT + sb - 1) / sb);
     const R kp = (R)(0.5 * P.rho * P.c * P.c), kc = (R)(0.5 * P.c), kr = (R)(0.5 / P.rho);
+    // flux adjoint: tiled kernel unless the per-cell one was asked for
+    const bool tiled = P.adjoint_variant != 1;
+    const int atz = 32;
+    const dim3 agrid((ctx->Nx - 6 + adjtile::TX - 1) / adjtile::TX, (ctx->Ny - 6 + 11) / 12, (ctx->Nz - 6 + atz - 1) / atz);
     for (int c = A.n_ckpt - 1; c >= 0; --c) {
         const long long n0 = A.step0 + (long long)c * A.every;
         const long long len = std::min<long long>(A.every, A.step0 + nsteps - n0);
@@ -332,37 +375,62 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
             const AdeState &X0 = A.xs[(size_t)q];
             if (ctx->n_rec > 0)
                 hipLaunchKernelGGL((k_receiver_adj<R>), dim3(1), dim3(64), 0, ctx->stream, (R *)A.lam[iNext][0],
-                                   ctx->d_rcell, ctx->n_rec, A.dj, nsteps, n - A.step0);
+                                   ctx->d_rcell, ctx->n_rec, A.dj, nsteps, n - A.step0, pidx, A.lam_sh[iNext]);
             if (ctx->nB > 0)
                 hipLaunchKernelGGL((k_ade_replay<S>), bgrid, dim3(sb), 0, ctx->stream, g, V0, V1, bd, ade, P.rho, P.c,
                                    P.dt, X0, A.rx1, A.rx2, scd);
-            if ((rc = zero_ade(ctx, A.lx[ix0])) || (rc = zero_ade(ctx, A.lx[ix1])) || (rc = zero_ade(ctx, A.lx[ix2])))
-                return rc;
             // stage table: (input set, output adjoint, input adjoint, ADE right state, lnew, lr,
             //               a, b, c, RK alpha / beta, source weight and sample)
             struct St {
-                FieldSet<S> vin; int lout, lin; const AdeState *xr; int lnew, lr;
+                FieldSet<S> vin; const double *vsh; int lout, lin; const AdeState *xr; int lnew, lr;
                 double a, b, c, alpha, beta, w; long long nn;
             };
+            const double *sh0 = np ? A.s0[(size_t)q] : nullptr, *sh1 = np ? A.s1[(size_t)q] : nullptr,
+                         *sh2 = np ? A.s2[(size_t)q] : nullptr;
             const St st[3] = {
-                {V0, iV1, iU0, &X0, ix1, ix0, 0.0, 1.0, 1.0, 0.0, 1.0, 1.0, 2 * n},
-                {V1, iV2, iV1, &A.rx1, ix2, ix1, 3.0, 1.0, 4.0, 0.75, 0.25, 0.25, 2 * n + 2},
-                {V2, iNext, iV2, &A.rx2, ix3, ix2, 1.0, 2.0, 3.0, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 2 * n + 1},
+                {V0, sh0, iV1, iU0, &X0, ix1, ix0, 0.0, 1.0, 1.0, 0.0, 1.0, 1.0, 2 * n},
+                {V1, sh1, iV2, iV1, &A.rx1, ix2, ix1, 3.0, 1.0, 4.0, 0.75, 0.25, 0.25, 2 * n + 2},
+                {V2, sh2, iNext, iV2, &A.rx2, ix3, ix2, 1.0, 2.0, 3.0, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 2 * n + 1},
             };
             for (int s = 2; s >= 0; --s) {
                 const St &t = st[s];
                 const FieldSet<R> Lo = lam_set<R>(A, t.lout), Li = lam_set<R>(A, t.lin), Lb = lam_set<R>(A, iU0);
-                if (ctx->nB > 0)
-                    hipLaunchKernelGGL((k_boundary_adj<S, R>), bgrid, dim3(sb), 0, ctx->stream, g, t.vin, bd, ade,
-                                       P.rho, P.c, P.dt, t.a, t.b, t.c, X0, *t.xr, A.lx[t.lnew], A.lx[ix0], A.lx[t.lr],
-                                       Lo.p, Lo.u, Lo.v, Lo.w, A.lD, A.G, A.P, scd);
+                // first writers of the step's ADE adjoints: stage 2 sets x0 (mid) and x2 (r),
+                // stage 1 sets x1 (r), stage 0 adds to x0
+                if (ctx->nB > 0) {
+                    if (s == 2)
+                        hipLaunchKernelGGL((k_boundary_adj<S, R, true, true>), bgrid, dim3(sb), 0, ctx->stream, g, t.vin,
+                                           bd, ade, P.rho, P.c, P.dt, t.a, t.b, t.c, X0, *t.xr, A.lx[t.lnew],
+                                           A.lx[ix0], A.lx[t.lr], Lo.p, Lo.u, Lo.v, Lo.w, A.lD, A.G, A.P, scd);
+                    else if (s == 1)
+                        hipLaunchKernelGGL((k_boundary_adj<S, R, false, true>), bgrid, dim3(sb), 0, ctx->stream, g,
+                                           t.vin, bd, ade, P.rho, P.c, P.dt, t.a, t.b, t.c, X0, *t.xr, A.lx[t.lnew],
+                                           A.lx[ix0], A.lx[t.lr], Lo.p, Lo.u, Lo.v, Lo.w, A.lD, A.G, A.P, scd);
+                    else
+                        hipLaunchKernelGGL((k_boundary_adj<S, R, false, false>), bgrid, dim3(sb), 0, ctx->stream, g,
+                                           t.vin, bd, ade, P.rho, P.c, P.dt, t.a, t.b, t.c, X0, *t.xr, A.lx[t.lnew],
+                                           A.lx[ix0], A.lx[t.lr], Lo.p, Lo.u, Lo.v, Lo.w, A.lD, A.G, A.P, scd);
+                }
                 if (ctx->n_src > 0)
                     hipLaunchKernelGGL((k_source_adj<R>), dim3((ctx->n_src + 63) / 64), dim3(64), 0, ctx->stream, Lo.p,
                                        ctx->d_scell, ctx->d_sval, ctx->d_us,
                                        ctx->src_opt == FASES_SRC_SINE ? ctx->n_samp : 0LL, ctx->n_src, t.nn,
-                                       t.w * coef0, A.gs, gus);
+                                       t.w * coef0, A.gs, gus, pidx, A.lam_sh[t.lout]);
                 const AdjK<R> K{kp, kc, kr, (R)(t.beta * P.dt / P.h), (R)t.alpha, (R)t.beta};
-                if (s == 2)
+                if (tiled) {
+                    if (s == 2)
+                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, false, false, true, 16>), agrid, dim3(64, 16), 0,
+                                           ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w,
+                                           ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz);
+                    else if (s == 1)
+                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, false, true, true, 16>), agrid, dim3(64, 16), 0,
+                                           ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w,
+                                           ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz);
+                    else
+                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, true, false, false, 16>), agrid, dim3(64, 16), 0,
+                                           ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w,
+                                           ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz);
+                } else if (s == 2)
                     hipLaunchKernelGGL((k_flux_adj_cell<S, R, false, false, true>), fgrid, fb, 0, ctx->stream, g,
                                        t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w, ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p,
                                        Lb.u, Lb.v, Lb.w, K);
@@ -374,9 +442,29 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
                     hipLaunchKernelGGL((k_flux_adj_cell<S, R, true, false, false>), fgrid, fb, 0, ctx->stream, g,
                                        t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w, ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p,
                                        Lb.u, Lb.v, Lb.w, K);
+                if (nap) {
+                    const AdjK<double> Kd{0.5 * P.rho * P.c * P.c, 0.5 * P.c, 0.5 / P.rho, t.beta * P.dt / P.h,
+                                          t.alpha, t.beta};
+                    const dim3 pg((unsigned)((nap + 63) / 64));
+                    if (s == 2)
+                        hipLaunchKernelGGL((k_flux_adj_patch<S, R, false, false, true>), pg, dim3(64), 0, ctx->stream,
+                                           g, t.vin, t.vsh, ctx->d_pidx, (int)np, scd, Lo.p, Lo.u, Lo.v, Lo.w,
+                                           A.lam_sh[t.lout], ctx->d_upd, A.ap_cells, A.ap_idx, nap, Li.p, Li.u, Li.v,
+                                           Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w, A.lam_sh[iU0], Kd);
+                    else if (s == 1)
+                        hipLaunchKernelGGL((k_flux_adj_patch<S, R, false, true, true>), pg, dim3(64), 0, ctx->stream,
+                                           g, t.vin, t.vsh, ctx->d_pidx, (int)np, scd, Lo.p, Lo.u, Lo.v, Lo.w,
+                                           A.lam_sh[t.lout], ctx->d_upd, A.ap_cells, A.ap_idx, nap, Li.p, Li.u, Li.v,
+                                           Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w, A.lam_sh[iU0], Kd);
+                    else
+                        hipLaunchKernelGGL((k_flux_adj_patch<S, R, true, false, false>), pg, dim3(64), 0, ctx->stream,
+                                           g, t.vin, t.vsh, ctx->d_pidx, (int)np, scd, Lo.p, Lo.u, Lo.v, Lo.w,
+                                           A.lam_sh[t.lout], ctx->d_upd, A.ap_cells, A.ap_idx, nap, Li.p, Li.u, Li.v,
+                                           Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w, A.lam_sh[iU0], Kd);
+                }
                 if (ctx->nB > 0)
                     hipLaunchKernelGGL((k_probe_gather<R>), tgrid, dim3(sb), 0, ctx->stream, A.nT, A.tgt, A.toff, A.eb,
-                                       A.ew, A.lD, Li.p, Li.u, Li.v, Li.w);
+                                       A.ew, A.lD, Li.p, Li.u, Li.v, Li.w, pidx, A.lam_sh[t.lin], nap);
             }
             std::swap(iNext, iU0);
             std::swap(ix3, ix0);

@@ -94,29 +94,13 @@ template <typename R> struct AdjK {
     R alpha, beta;      // stage combination out = alpha U0 + beta (V + dt L)
 };
 
-// Flux-stage adjoint, one thread per cell of [3, N-3)^3:
-//   lin  (=|+=) beta Lout + (flux part)       lbase (=|+=) alpha Lout
-// Lout is the adjoint of the stage output after the boundary adjoint; only updated
-// cells (upd != 0) carry it into the stage. ACC_IN / ACC_BASE select += over =.
-template <typename S, typename R, bool ACC_IN, bool ACC_BASE, bool HAS_BASE>
-__global__ void __launch_bounds__(256)
-k_flux_adj_cell(Grid g, FieldSet<S> V, Scl<R> sc, const R *__restrict__ L0, const R *__restrict__ L1,
-                const R *__restrict__ L2, const R *__restrict__ L3, const uint8_t *__restrict__ upd, R *lin0,
-                R *lin1, R *lin2, R *lin3, R *lb0, R *lb1, R *lb2, R *lb3, AdjK<R> K) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 3;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 3;
-    const int k = blockIdx.z + 3;
-    if (i >= g.Nx - 3 || j >= g.Ny - 3 || k >= g.Nz - 3) return;
+// Flux-part adjoint of one cell C = (i, j, k): the sum over the three axes and the five
+// cells Cq whose WENO pair windows contain C of their pair VJPs at C's slot. vv(f, q) is
+// the forward value of field f at cell q, lt(f, q) the scaled, masked output adjoint.
+template <typename R, typename VA, typename LA>
+__device__ inline void flux_adj_sum(const Grid &g, int i, int j, int k, const VA &vv, const LA &lt, const AdjK<R> &K,
+                                    R acc[4]) {
     const long long C = (long long)i + (long long)g.Nx * j + g.nxy * k;
-    const R *Lf[4] = {L0, L1, L2, L3};
-    const S *Vf[4] = {V.p, V.u, V.v, V.w};
-    // masked, scaled output adjoint of a cell
-    auto lt = [&](int f, long long q) -> R { return upd[q] ? K.bdh * Lf[f][q] : R(0); };
-    auto vv = [&](int f, long long q) -> R { return Conv<S, R>::ld(Vf[f][q], sc.s[f]); };
-    R acc[4];
-    const bool me = upd[C] != 0;
-#pragma unroll
-    for (int f = 0; f < 4; ++f) acc[f] = me ? K.beta * Lf[f][C] : R(0);
     const long long st[3] = {1, (long long)g.Nx, g.nxy};
     const int pos[3] = {i, j, k};
     const int nax[3] = {g.Nx, g.Ny, g.Nz};
@@ -145,6 +129,31 @@ k_flux_adj_cell(Grid g, FieldSet<S> V, Scl<R> sc, const R *__restrict__ L0, cons
             acc[fu] += gu[q + 2];
         }
     }
+}
+
+// Flux-stage adjoint, one thread per cell of [3, N-3)^3:
+//   lin  (=|+=) beta Lout + (flux part)       lbase (=|+=) alpha Lout
+// Lout is the adjoint of the stage output after the boundary adjoint; only updated
+// cells (upd != 0) carry it into the stage. ACC_IN / ACC_BASE select += over =.
+template <typename S, typename R, bool ACC_IN, bool ACC_BASE, bool HAS_BASE>
+__global__ void __launch_bounds__(256)
+k_flux_adj_cell(Grid g, FieldSet<S> V, Scl<R> sc, const R *__restrict__ L0, const R *__restrict__ L1,
+                const R *__restrict__ L2, const R *__restrict__ L3, const uint8_t *__restrict__ upd, R *lin0,
+                R *lin1, R *lin2, R *lin3, R *lb0, R *lb1, R *lb2, R *lb3, AdjK<R> K) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 3;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 3;
+    const int k = blockIdx.z + 3;
+    if (i >= g.Nx - 3 || j >= g.Ny - 3 || k >= g.Nz - 3) return;
+    const long long C = (long long)i + (long long)g.Nx * j + g.nxy * k;
+    const R *Lf[4] = {L0, L1, L2, L3};
+    const S *Vf[4] = {V.p, V.u, V.v, V.w};
+    auto lt = [&](int f, long long q) -> R { return upd[q] ? K.bdh * Lf[f][q] : R(0); };
+    auto vv = [&](int f, long long q) -> R { return Conv<S, R>::ld(Vf[f][q], sc.s[f]); };
+    const bool me = upd[C] != 0;
+    R acc[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) acc[f] = me ? K.beta * Lf[f][C] : R(0);
+    flux_adj_sum<R>(g, i, j, k, vv, lt, K, acc);
     R *li[4] = {lin0, lin1, lin2, lin3};
 #pragma unroll
     for (int f = 0; f < 4; ++f) li[f][C] = ACC_IN ? li[f][C] + acc[f] : acc[f];
@@ -155,6 +164,294 @@ k_flux_adj_cell(Grid g, FieldSet<S> V, Scl<R> sc, const R *__restrict__ L0, cons
             const R v = me ? K.alpha * Lf[f][C] : R(0);
             lb[f][C] = ACC_BASE ? lb[f][C] + v : v;
         }
+    }
+}
+
+// FP64 adjoint of the cells near point sources (reduced-precision storage): the forward
+// updates them in FP64 from FP64 shadows, and so does the adjoint. Reads the forward
+// states from the stage input's shadow (forward patch map vidx) and the output adjoint
+// from its FP64 adjoint shadow (adjoint patch map pidx; stored copies elsewhere); writes
+// the FP64 adjoint shadows of the input (and base) set and their stored copies, after
+// the grid kernel has run.
+template <typename S, typename R, bool ACC_IN, bool ACC_BASE, bool HAS_BASE>
+__global__ void __launch_bounds__(64)
+k_flux_adj_patch(Grid g, FieldSet<S> V, const double *__restrict__ vsh, const int *__restrict__ vidx, int nv,
+                 Scl<double> sc, const R *__restrict__ L0, const R *__restrict__ L1, const R *__restrict__ L2,
+                 const R *__restrict__ L3, const double *__restrict__ lsh_out, const uint8_t *__restrict__ upd,
+                 const int *__restrict__ cells, const int *__restrict__ pidx, int np, R *lin0, R *lin1, R *lin2,
+                 R *lin3, double *lsh_in, R *lb0, R *lb1, R *lb2, R *lb3, double *lsh_base, AdjK<double> K) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    const int C = cells[t];
+    const int i = C % g.Nx, j = (C / g.Nx) % g.Ny, k = (int)(C / g.nxy);
+    const R *Lf[4] = {L0, L1, L2, L3};
+    const S *Vf[4] = {V.p, V.u, V.v, V.w};
+    auto lraw = [&](int f, long long q) -> double {
+        const int w = pidx[q];
+        return w >= 0 ? lsh_out[(long long)f * np + w] : (double)Lf[f][q];
+    };
+    auto lt = [&](int f, long long q) -> double { return upd[q] ? K.bdh * lraw(f, q) : 0.0; };
+    // forward values: FP64 shadows of the forward patch (index vidx, nv cells)
+    auto vv = [&](int f, long long q) -> double {
+        const int w = vidx[q];
+        return w >= 0 ? vsh[(long long)f * nv + w] : Conv<S, double>::ld(Vf[f][q], sc.s[f]);
+    };
+    double acc[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) acc[f] = K.beta * lraw(f, C);
+    flux_adj_sum<double>(g, i, j, k, vv, lt, K, acc);
+    R *li[4] = {lin0, lin1, lin2, lin3};
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        double *o = lsh_in + (long long)f * np + t;
+        const double v = ACC_IN ? *o + acc[f] : acc[f];
+        *o = v;
+        li[f][C] = (R)v;
+    }
+    if (HAS_BASE) {
+        R *lb[4] = {lb0, lb1, lb2, lb3};
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            double *o = lsh_base + (long long)f * np + t;
+            const double v = (ACC_BASE ? *o : 0.0) + K.alpha * lraw(f, C);
+            *o = v;
+            lb[f][C] = (R)v;
+        }
+    }
+}
+
+// Tiled flux-stage adjoint. Block = 64 lanes x NW waves; lane l holds column
+// X0 - 2 + l and wave w row Y0 - 2 + w, marching over planes in z. Every cell's two
+// WENO pair VJPs per axis are evaluated once; their five window contributions reach
+// their cells through lane shuffles (x), an LDS exchange (y) and a ring of five plane
+// accumulators (z). Outputs: the inner 60 columns x (NW - 4) rows, plane k - 2 once the
+// pipeline has passed plane k + 2... i.e. two planes behind the march.
+namespace adjtile {
+constexpr int TX = 60;       // output columns per block
+constexpr int VX = 68;       // V tile columns: X0-4 .. X0+63
+constexpr int LXA = 66;      // adjoint tile columns: X0-3 .. X0+62
+}  // namespace adjtile
+
+template <typename R, int NW> struct AdjLds {
+    R vp[NW + 4][adjtile::VX];   // rows Y0-4 .. Y0+NW-1
+    R vu[NW][adjtile::VX];
+    R vv[NW + 4][64];
+    R lp[NW + 2][adjtile::LXA];  // rows Y0-3 .. Y0+NW-2 (scaled, masked output adjoint)
+    R lu[NW][adjtile::LXA];
+    R lv[NW + 2][64];
+    R gy[2][5][NW][64];          // y-pair contributions of every row: p, v
+};
+
+template <typename S, typename R, bool ACC_IN, bool ACC_BASE, bool HAS_BASE, int NW>
+__global__ void __launch_bounds__(64 * NW)
+k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__restrict__ L0,
+                 const R *__restrict__ L1, const R *__restrict__ L2, const R *__restrict__ L3,
+                 const uint8_t *__restrict__ upd, R *lin0, R *lin1, R *lin2, R *lin3, R *lb0, R *lb1, R *lb2,
+                 R *lb3, AdjK<R> K, int tz) {
+    using namespace adjtile;
+    __shared__ AdjLds<R, NW> T;
+    typedef Conv<S, R> CV;
+    const int lx = threadIdx.x, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int tid = wy * 64 + lx;
+    const int X0 = 3 + TX * blockIdx.x, Y0 = 3 + (NW - 4) * blockIdx.y, k0 = 3 + tz * blockIdx.z;
+    const int kend = min(k0 + tz, Nz - 3);
+    const long long nxy = (long long)Nx * Ny;
+    const int x = X0 - 2 + lx, y = Y0 - 2 + wy;
+    const int xc = max(0, min(x, Nx - 1)), yc = max(0, min(y, Ny - 1));
+    const long long col = xc + (long long)Nx * yc;
+    const bool out_row = (wy >= 2 && wy <= NW - 3) && y < Ny - 3;
+    const bool out_cell = out_row && lx >= 2 && lx <= 61 && x < Nx - 3;
+    const R bdh = K.bdh;
+    auto plane = [&](int k) -> long long { return nxy * (long long)max(0, min(k, Nz - 1)); };
+    auto lt = [&](const R *a, long long q) -> R { return upd[q] ? bdh * a[q] : R(0); };
+
+    // halo item of this thread (see the tile shapes above)
+    int h_kind = -1, h_r = 0, h_c = 0;
+    long long h_off = 0;
+    {
+        int t = tid;
+        const int n0 = 4 * VX + 4 * NW, n1 = 4 * NW, n2 = 4 * 64, n3 = 2 * LXA + 2 * NW, n4 = 2 * NW, n5 = 2 * 64;
+        const int rA[4] = {0, 1, NW + 2, NW + 3};
+        const int cA[4] = {0, 1, VX - 2, VX - 1};
+        if (t < n0) {
+            h_kind = 0;
+            if (t < 4 * VX) { h_r = rA[t / VX]; h_c = t % VX; }
+            else { t -= 4 * VX; h_c = cA[t / NW]; h_r = 2 + t % NW; }
+        } else if ((t -= n0) < n1) {
+            h_kind = 1; h_c = cA[t / NW]; h_r = t % NW;
+        } else if ((t -= n1) < n2) {
+            h_kind = 2; h_r = rA[t / 64]; h_c = t % 64;
+        } else if ((t -= n2) < n3) {
+            h_kind = 3;
+            if (t < 2 * LXA) { h_r = (t / LXA) ? NW + 1 : 0; h_c = t % LXA; }
+            else { t -= 2 * LXA; h_c = (t / NW) ? LXA - 1 : 0; h_r = 1 + t % NW; }
+        } else if ((t -= n3) < n4) {
+            h_kind = 4; h_c = (t / NW) ? LXA - 1 : 0; h_r = t % NW;
+        } else if ((t -= n4) < n5) {
+            h_kind = 5; h_r = (t / 64) ? NW + 1 : 0; h_c = t % 64;
+        }
+        if (h_kind >= 0) {
+            int gx, gy;
+            if (h_kind <= 2) {          // V tiles: vp (r, c) -> (Y0-4+r, X0-4+c); vu rows are wy; vv cols are lanes
+                gx = (h_kind == 2) ? X0 - 2 + h_c : X0 - 4 + h_c;
+                gy = (h_kind == 1) ? Y0 - 2 + h_r : Y0 - 4 + h_r;
+            } else {                    // adjoint tiles: lp (r, c) -> (Y0-3+r, X0-3+c)
+                gx = (h_kind == 5) ? X0 - 2 + h_c : X0 - 3 + h_c;
+                gy = (h_kind == 4) ? Y0 - 2 + h_r : Y0 - 3 + h_r;
+            }
+            gx = max(0, min(gx, Nx - 1));
+            gy = max(0, min(gy, Ny - 1));
+            h_off = gx + (long long)Nx * gy;
+        }
+    }
+
+    // z windows of the own column: V p, w at k-2 .. k+2, scaled output adjoint at k-1 .. k+1
+    R zp[5], zw[5], ap[3], aw[3];
+    R za[5] = {0, 0, 0, 0, 0}, zb[5] = {0, 0, 0, 0, 0};        // z contributions to planes k-2 .. k+2 (p, w)
+    R xa[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};            // x/y contributions, planes k-2 .. k (p, u, v)
+    {
+        const int kb = k0 - 2;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const long long pq = plane(kb - 2 + q) + col;
+            zp[q] = CV::ld(V.p[pq], sc.s[0]);
+            zw[q] = CV::ld(V.w[pq], sc.s[3]);
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int kq = kb - 1 + q;
+            const long long pq = plane(kq) + col;
+            const bool inside = kq >= 0 && kq < Nz;
+            ap[q] = inside ? lt(L0, pq) : R(0);
+            aw[q] = inside ? lt(L3, pq) : R(0);
+        }
+    }
+    for (int k = k0 - 2; k <= kend + 1; ++k) {
+        const bool xy = (k >= k0 && k < kend);                 // block-uniform
+        // lead loads for the next plane's windows
+        R np_ = 0, nw_ = 0, nap = 0, naw = 0;
+        {
+            const long long p3 = plane(k + 3) + col;
+            np_ = CV::ld(V.p[p3], sc.s[0]);
+            nw_ = CV::ld(V.w[p3], sc.s[3]);
+            if (k + 2 < Nz) {
+                const long long p2 = nxy * (long long)(k + 2) + col;
+                nap = lt(L0, p2);
+                naw = lt(L3, p2);
+            }
+        }
+        if (xy) {
+            const long long pk = nxy * (long long)k;
+            T.vp[wy + 2][lx + 2] = zp[2];
+            T.vu[wy][lx + 2] = CV::ld(V.u[pk + col], sc.s[1]);
+            T.vv[wy + 2][lx] = CV::ld(V.v[pk + col], sc.s[2]);
+            T.lp[wy + 1][lx + 1] = ap[1];
+            T.lu[wy][lx + 1] = lt(L1, pk + col);
+            T.lv[wy + 1][lx] = lt(L2, pk + col);
+            if (h_kind >= 0) {
+                const long long q = pk + h_off;
+                switch (h_kind) {
+                case 0: T.vp[h_r][h_c] = CV::ld(V.p[q], sc.s[0]); break;
+                case 1: T.vu[h_r][h_c] = CV::ld(V.u[q], sc.s[1]); break;
+                case 2: T.vv[h_r][h_c] = CV::ld(V.v[q], sc.s[2]); break;
+                case 3: T.lp[h_r][h_c] = lt(L0, q); break;
+                case 4: T.lu[h_r][h_c] = lt(L1, q); break;
+                default: T.lv[h_r][h_c] = lt(L2, q); break;
+                }
+            }
+            __syncthreads();
+            // ---- y pairs of every row
+            {
+                const R a_m = T.lp[wy][lx + 1], a_c = T.lp[wy + 1][lx + 1], a_p = T.lp[wy + 2][lx + 1];
+                const R b_m = T.lv[wy][lx], b_c = T.lv[wy + 1][lx], b_p = T.lv[wy + 2][lx];
+                R mpL, mpR, mvL, mvR, t0, t1, t2, t3;
+                godunov_vjp(a_p - a_c, b_p - b_c, K.kp, K.kc, K.kr, mpL, t0, mvL, t1);
+                godunov_vjp(a_c - a_m, b_c - b_m, K.kp, K.kc, K.kr, t2, mpR, t3, mvR);
+                R gp[5], gv[5];
+                weno_pair_vjp(T.vp[wy][lx + 2], T.vp[wy + 1][lx + 2], T.vp[wy + 2][lx + 2], T.vp[wy + 3][lx + 2],
+                              T.vp[wy + 4][lx + 2], mpL, mpR, gp);
+                weno_pair_vjp(T.vv[wy][lx], T.vv[wy + 1][lx], T.vv[wy + 2][lx], T.vv[wy + 3][lx], T.vv[wy + 4][lx],
+                              mvL, mvR, gv);
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    T.gy[0][q][wy][lx] = gp[q];
+                    T.gy[1][q][wy][lx] = gv[q];
+                }
+            }
+            // ---- x pairs of the output rows (all lanes of those waves)
+            R cxp = 0, cxu = 0;
+            if (out_row) {
+                const R a_m = T.lp[wy + 1][lx], a_c = T.lp[wy + 1][lx + 1], a_p = T.lp[wy + 1][lx + 2];
+                const R b_m = T.lu[wy][lx], b_c = T.lu[wy][lx + 1], b_p = T.lu[wy][lx + 2];
+                R mpL, mpR, muL, muR, t0, t1, t2, t3;
+                godunov_vjp(a_p - a_c, b_p - b_c, K.kp, K.kc, K.kr, mpL, t0, muL, t1);
+                godunov_vjp(a_c - a_m, b_c - b_m, K.kp, K.kc, K.kr, t2, mpR, t3, muR);
+                R gp[5], gu[5];
+                weno_pair_vjp(T.vp[wy + 2][lx], T.vp[wy + 2][lx + 1], T.vp[wy + 2][lx + 2], T.vp[wy + 2][lx + 3],
+                              T.vp[wy + 2][lx + 4], mpL, mpR, gp);
+                weno_pair_vjp(T.vu[wy][lx], T.vu[wy][lx + 1], T.vu[wy][lx + 2], T.vu[wy][lx + 3], T.vu[wy][lx + 4],
+                              muL, muR, gu);
+                cxp = gp[2] + __shfl_up(gp[3], 1, 64) + __shfl_up(gp[4], 2, 64) + __shfl_down(gp[1], 1, 64) +
+                      __shfl_down(gp[0], 2, 64);
+                cxu = gu[2] + __shfl_up(gu[3], 1, 64) + __shfl_up(gu[4], 2, 64) + __shfl_down(gu[1], 1, 64) +
+                      __shfl_down(gu[0], 2, 64);
+            }
+            __syncthreads();
+            if (out_row) {
+                R cyp = 0, cyv = 0;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    cyp += T.gy[0][q][wy + 2 - q][lx];
+                    cyv += T.gy[1][q][wy + 2 - q][lx];
+                }
+                xa[2][0] = cxp + cyp;
+                xa[2][1] = cxu;
+                xa[2][2] = cyv;
+            }
+        }
+        // ---- z pair of cell k (output rows)
+        if (out_row && k >= 2 && k <= Nz - 3) {
+            R mpL, mpR, mwL, mwR, t0, t1, t2, t3;
+            godunov_vjp(ap[2] - ap[1], aw[2] - aw[1], K.kp, K.kc, K.kr, mpL, t0, mwL, t1);
+            godunov_vjp(ap[1] - ap[0], aw[1] - aw[0], K.kp, K.kc, K.kr, t2, mpR, t3, mwR);
+            R gp[5], gw[5];
+            weno_pair_vjp(zp[0], zp[1], zp[2], zp[3], zp[4], mpL, mpR, gp);
+            weno_pair_vjp(zw[0], zw[1], zw[2], zw[3], zw[4], mwL, mwR, gw);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) { za[q] += gp[q]; zb[q] += gw[q]; }
+        }
+        // ---- plane k - 2 is complete
+        const int m = k - 2;
+        if (out_cell && m >= k0 && m < kend) {
+            const long long C = nxy * (long long)m + col;
+            const bool me = upd[C] != 0;
+            const R o0 = L0[C], o1 = L1[C], o2 = L2[C], o3 = L3[C];
+            const R v0 = xa[0][0] + za[0] + (me ? K.beta * o0 : R(0));
+            const R v1 = xa[0][1] + (me ? K.beta * o1 : R(0));
+            const R v2 = xa[0][2] + (me ? K.beta * o2 : R(0));
+            const R v3 = zb[0] + (me ? K.beta * o3 : R(0));
+            lin0[C] = ACC_IN ? lin0[C] + v0 : v0;
+            lin1[C] = ACC_IN ? lin1[C] + v1 : v1;
+            lin2[C] = ACC_IN ? lin2[C] + v2 : v2;
+            lin3[C] = ACC_IN ? lin3[C] + v3 : v3;
+            if (HAS_BASE) {
+                const R w0 = me ? K.alpha * o0 : R(0), w1 = me ? K.alpha * o1 : R(0);
+                const R w2 = me ? K.alpha * o2 : R(0), w3 = me ? K.alpha * o3 : R(0);
+                lb0[C] = ACC_BASE ? lb0[C] + w0 : w0;
+                lb1[C] = ACC_BASE ? lb1[C] + w1 : w1;
+                lb2[C] = ACC_BASE ? lb2[C] + w2 : w2;
+                lb3[C] = ACC_BASE ? lb3[C] + w3 : w3;
+            }
+        }
+        // ---- advance the rings
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { za[q] = za[q + 1]; zb[q] = zb[q + 1]; zp[q] = zp[q + 1]; zw[q] = zw[q + 1]; }
+        za[4] = 0; zb[4] = 0;
+        zp[4] = np_; zw[4] = nw_;
+        ap[0] = ap[1]; ap[1] = ap[2]; ap[2] = nap;
+        aw[0] = aw[1]; aw[1] = aw[2]; aw[2] = naw;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) { xa[0][f] = xa[1][f]; xa[1][f] = xa[2][f]; xa[2][f] = 0; }
     }
 }
 
@@ -227,12 +524,14 @@ k_ade_replay(Grid g, FieldSet<S> V0, FieldSet<S> V1, Bnd<double> bd, Ade ade, do
 }
 
 // Boundary + ADE adjoint of one stage (a, b, c) with state buffers mid = x0 and right
-// xr; lnew is the adjoint of the stage's new state (complete), lmid / lr accumulate.
+// xr; lnew is the adjoint of the stage's new state (complete), lmid / lr accumulate
+// (SET_MID / SET_R: first writer of that buffer in the step, which writes instead of
+// adding, so the buffers need no clearing).
 // Lout (adjoint of the stage output) is read and replaced by (1 - chi) Lout on the
 // boundary cell; lD[4 b + f] receives the adjoint of the probe values (p, u, v, w);
 // G[P b + .] accumulates the material-parameter gradient of the cell, layout
 // (Rinf, A[K], zeta[K], B[L], C[L], alpha[L], beta[L]).
-template <typename S, typename R>
+template <typename S, typename R, bool SET_MID, bool SET_R>
 __global__ void __launch_bounds__(256)
 k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd<double> bd, Ade ade, double rho, double c, double dt, double as,
                double bs, double cs, AdeState xmid, AdeState xr, AdeState lnew, AdeState lmid, AdeState lr,
@@ -271,8 +570,9 @@ k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd<double> bd, Ade ade, double rho, dou
         xn /= cs;
         const double ln = lnew.phi[s] + ade.A[mi] * lwin;
         Gb[1 + kk] += lwin * xn;
-        lmid.phi[s] += (as / cs) * ln;
-        lr.phi[s] += (bs / cs) * (1.0 - dt * z) * ln;
+        // stage 0 has mid == r: both terms land in one buffer, mid first
+        lmid.phi[s] = (SET_MID ? 0.0 : lmid.phi[s]) + (as / cs) * ln;
+        lr.phi[s] = (SET_R ? 0.0 : lr.phi[s]) + (bs / cs) * (1.0 - dt * z) * ln;
         lwout += f * ln;
         Gb[1 + K + kk] += -f * x_r * ln;
     }
@@ -288,10 +588,10 @@ k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd<double> bd, Ade ade, double rho, dou
         const double l1 = lnew.p1[s] + ade.B[mi] * lwin, l2 = lnew.p2[s] + ade.C[mi] * lwin;
         Gb[1 + 2 * K + l] += lwin * y1n;
         Gb[1 + 2 * K + L + l] += lwin * y2n;
-        lmid.p1[s] += (as / cs) * l1;
-        lmid.p2[s] += (as / cs) * l2;
-        lr.p1[s] += (bs / cs) * (l1 * (1.0 - dt * al) + l2 * (dt * be));
-        lr.p2[s] += (bs / cs) * (l1 * (-dt * be) + l2 * (1.0 - dt * al));
+        lmid.p1[s] = (SET_MID ? 0.0 : lmid.p1[s]) + (as / cs) * l1;
+        lmid.p2[s] = (SET_MID ? 0.0 : lmid.p2[s]) + (as / cs) * l2;
+        lr.p1[s] = (SET_R ? 0.0 : lr.p1[s]) + (bs / cs) * (l1 * (1.0 - dt * al) + l2 * (dt * be));
+        lr.p2[s] = (SET_R ? 0.0 : lr.p2[s]) + (bs / cs) * (l1 * (-dt * be) + l2 * (1.0 - dt * al));
         lwout += f * l1;
         Gb[1 + 2 * K + 2 * L + l] += f * (-y1 * l1 - y2 * l2);
         Gb[1 + 2 * K + 3 * L + l] += f * (-y2 * l1 + y1 * l2);
@@ -310,7 +610,7 @@ k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd<double> bd, Ade ade, double rho, dou
 template <typename R>
 __global__ void k_probe_gather(int nT, const int *__restrict__ tgt, const int *__restrict__ off,
                                const int *__restrict__ eb, const double *__restrict__ ew, const double *__restrict__ lD,
-                               R *L0, R *L1, R *L2, R *L3) {
+                               R *L0, R *L1, R *L2, R *L3, const int *__restrict__ pidx, double *lsh, int np) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nT) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -323,6 +623,13 @@ __global__ void k_probe_gather(int nT, const int *__restrict__ tgt, const int *_
         s3 += w * d[3];
     }
     const int C = tgt[t];
+    const int w = pidx ? pidx[C] : -1;
+    if (w >= 0) {                       // cell with an FP64 adjoint shadow
+        double *o = lsh + w;
+        o[0] += s0; o[np] += s1; o[2 * (long long)np] += s2; o[3 * (long long)np] += s3;
+        L0[C] = (R)o[0]; L1[C] = (R)o[np]; L2[C] = (R)o[2 * (long long)np]; L3[C] = (R)o[3 * (long long)np];
+        return;
+    }
     L0[C] = (R)((double)L0[C] + s0);
     L1[C] = (R)((double)L1[C] + s1);
     L2[C] = (R)((double)L2[C] + s2);
@@ -333,10 +640,13 @@ __global__ void k_probe_gather(int nT, const int *__restrict__ tgt, const int *_
 template <typename R>
 __global__ void k_source_adj(const R *__restrict__ Lp, const int *__restrict__ cell, const double *__restrict__ sval,
                              const double *__restrict__ u_s, long long us_stride, int n_src, long long nn, double coef,
-                             double *__restrict__ gs, double *__restrict__ gus) {
+                             double *__restrict__ gs, double *__restrict__ gus, const int *__restrict__ pidx,
+                             const double *__restrict__ lsh_p) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_src) return;
-    const double lp = (double)Lp[cell[s]];
+    const int C = cell[s];
+    const int w = pidx ? pidx[C] : -1;
+    const double lp = w >= 0 ? lsh_p[w] : (double)Lp[C];
     const long long idx = (long long)s * us_stride + nn;
     gs[s] += lp * coef * u_s[idx];
     if (gus) atomicAdd(gus + idx, lp * coef * sval[s]);
@@ -345,9 +655,18 @@ __global__ void k_source_adj(const R *__restrict__ Lp, const int *__restrict__ c
 // IR misfit injection: Lp[rec_r] += dJ/dir[r][n] (sequential, receivers may share a cell).
 template <typename R>
 __global__ void k_receiver_adj(R *Lp, const int *__restrict__ cell, int n_rec, const double *__restrict__ dj,
-                               long long stride, long long n) {
+                               long long stride, long long n, const int *__restrict__ pidx, double *lsh_p) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int r = 0; r < n_rec; ++r) Lp[cell[r]] = (R)((double)Lp[cell[r]] + dj[r * stride + n]);
+    for (int r = 0; r < n_rec; ++r) {
+        const int C = cell[r];
+        const int w = pidx ? pidx[C] : -1;
+        if (w >= 0) {
+            lsh_p[w] += dj[r * stride + n];
+            Lp[C] = (R)lsh_p[w];
+        } else {
+            Lp[C] = (R)((double)Lp[C] + dj[r * stride + n]);
+        }
+    }
 }
 
 }  // namespace fasesdev

@@ -57,6 +57,7 @@ int fases_params_default(fases_params *p) {
     p->use_graph = 0;
     p->kernel_variant = 0;
     p->tile_z = 0;
+    p->adjoint_variant = 0;
     return FASES_OK;
 }
 

@@ -254,6 +254,7 @@ class SolverParams:
     use_graph: bool = False
     kernel_variant: int = 0             # 0 auto, 1 per-cell (reference order), 2 tiled
     tile_z: int = 0                     # tiled kernel z planes per block; 0 = autotune
+    adjoint_variant: int = 0            # flux adjoint: 0 tiled, 1 per cell
 
     def to_c(self) -> _lib.fases_params:
         p = _lib.fases_params()
@@ -271,6 +272,7 @@ class SolverParams:
         p.use_graph = int(self.use_graph)
         p.kernel_variant = int(self.kernel_variant)
         p.tile_z = int(self.tile_z)
+        p.adjoint_variant = int(self.adjoint_variant)
         return p
 
 

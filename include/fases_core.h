@@ -73,6 +73,8 @@ typedef struct fases_params {
                                2 tiled kernel (faces reconstructed once, LDS/shuffles) */
     int    tile_z;          /* tiled kernel: z planes per block; 0 = autotune at the
                                first fases_run (times 16, 32, 64, 128 on the real grid) */
+    int    adjoint_variant; /* flux adjoint of fases_adjoint: 0 tiled (each WENO pair VJP
+                               once), 1 per cell (neighbours' pairs re-evaluated)        */
 } fases_params;
 
 /* Fills *p with the reference defaults (FP64, device 0, PML depth 20, ...). */

@@ -210,3 +210,49 @@ def test_adjoint_rejects_pml_and_fp16():
     with pytest.raises(FasesError, match="adjoint_forward"):
         sv.adjoint(np.zeros((1, 10)))
     sv.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9)])
+def test_tiled_adjoint_equals_per_cell(prec, tol):
+    """The tiled flux adjoint (each pair VJP once, gathered by shuffles / LDS / z ring) and
+    the per-cell one (every cell re-evaluates its neighbours' pairs) differ only in the
+    order of the sums; the room spans several tiles on every axis (tile seams). FP64
+    only: 60 steps into a large room the gradient is carried by the wave front's tail,
+    where the WENO-Z weights switch sharply, and FP32 rounding moves it by percents
+    (the FP32 bar is set on a settled misfit in test_fp32_gradients_close_to_fp64)."""
+    announce("tiled vs per-cell adjoint %s" % prec)
+    wl = W.ade_room(room=(70, 30, 75), h=0.02, nsteps=60, seed=8, n_rec=4, n_types=2, K=2, L=1, pml_x_hi=False)
+    out = []
+    for av in (0, 1):
+        w2 = dataclasses.replace(wl, params=dataclasses.replace(wl.params, adjoint_variant=av))
+        sv = W.make_solver(w2, prec)
+        ir = sv.adjoint_forward(wl.nsteps, 16)
+        # J = sum ir^2 / 2: a misfit-like weight (random weights on every sample make the
+        # gradient a sum of large cancelling terms, which FP32 cannot resolve)
+        out.append(sv.adjoint(ir))
+        sv.close()
+    (ga, sa, _), (gb, sb, _) = out
+    scale = np.abs(gb).max(axis=1, keepdims=True)
+    print("max rel diff: materials %.3e, drive %.3e" % ((np.abs(ga - gb) / scale).max(),
+                                                        (np.abs(sa - sb) / np.abs(sb)).max()))
+    assert np.all(np.abs(ga - gb) <= tol * scale)
+    assert np.all(np.abs(sa - sb) <= tol * np.abs(sb))
+
+
+def test_fp32_drive_gradient_near_point_source():
+    """FP32 storage: the cells around a point source are advanced in FP64 from FP64
+    shadows, and their adjoint is too (without it the FP32 drive gradient of a
+    source-dominated misfit is wrong by factors). J = sum ir^2, so dJ/dsval is close to
+    2 J / sval (the IR is nearly linear in the drive)."""
+    announce("fp32 drive gradient with the FP64 adjoint patch")
+    wl = W.ade_room(room=(60, 50, 44), h=0.02, nsteps=300, seed=13, n_rec=4, n_types=2, K=2, L=1, pml_x_hi=False)
+    gs = {}
+    for prec in ("fp32", "fp64"):
+        sv = W.make_solver(wl, prec)
+        ir = sv.adjoint_forward(wl.nsteps, 64)
+        _, g, _ = sv.adjoint(2 * ir)
+        gs[prec] = (g[0], 2 * float((ir ** 2).sum()) / wl.src_sval[0])
+        sv.close()
+    print("dJ/dsval fp32 %.8e fp64 %.8e (2J/sval %.8e)" % (gs["fp32"][0], gs["fp64"][0], gs["fp64"][1]))
+    assert abs(gs["fp32"][0] - gs["fp64"][0]) <= 1e-4 * abs(gs["fp64"][0])
+    assert abs(gs["fp64"][0] - gs["fp64"][1]) <= 0.02 * abs(gs["fp64"][1])
