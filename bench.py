@@ -43,13 +43,15 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16"])
     ap.add_argument("--room", default="512,384,256")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline sample (0: skip)")
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=32, help="jobs of the batched benchmark (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=3000, help="steps of the adjoint benchmark (0: skip)")
+    ap.add_argument("--tile-z", type=int, default=0, help="flux kernel z depth per block (0: autotune)")
+    ap.add_argument("--tile-rows", type=int, default=0, help="flux kernel waves per block, 8 or 16 (0: autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the FVM core has no multi-GPU path in this build "
@@ -62,6 +64,9 @@ def main() -> int:
     total = args.warmup + args.steps
     t0 = time.time()
     wl = W.config(2, room=room, nsteps=total, precision=args.precision)
+    if args.tile_z or args.tile_rows:
+        import dataclasses
+        wl = dataclasses.replace(wl, params=dataclasses.replace(wl.params, tile_z=args.tile_z, tile_rows=args.tile_rows))
     sc = wl.scene
     log("scene %s padded %s built in %.1f s" % (room, sc.dims, time.time() - t0))
 

@@ -45,6 +45,7 @@ class fases_params(C.Structure):
         ("kernel_variant", C.c_int),
         ("tile_z", C.c_int),
         ("adjoint_variant", C.c_int),
+        ("tile_rows", C.c_int),
     ]
 
 

@@ -81,14 +81,15 @@ def free_mask(K: int, L: int, n: int) -> np.ndarray:
 
 
 def lbfgs_recover(problem: RoomProblem, target_ir: np.ndarray, theta0: np.ndarray, sval0: np.ndarray,
-                  iterations: int = 50, lr: float = 0.1, warm: int = 3, log=None):
+                  iterations: int = 50, lr: float = 0.02, warm: int = 3, log=None):
     """Fits theta = theta0 * exp(u) (multiplicative, so signs and scales are kept; the pair
     frequencies stay fixed) and sval = sval0 * exp(v) to the target IRs by L-BFGS on
     J = sum (ir - target)^2. Returns (theta, sval, history of J per iteration).
     The line search starts at lr times the quasi-Newton step for the first `warm`
-    iterations (a full first step can push a wall past |R| = 1, where the room becomes
-    unstable), at the full step afterwards; a trial point whose run goes non-finite
-    counts as a very large misfit so the line search backs off."""
+    iterations (the first step is then at most lr per log-parameter: a full step can
+    push a wall past |R| = 1, where the room becomes unstable), at the full step
+    afterwards; a trial point whose run goes non-finite counts as 1e3 times the first
+    misfit with a zero gradient, so the line search backs off."""
     th0 = torch.from_numpy(np.asarray(theta0, np.float64))
     s0 = torch.from_numpy(np.asarray(sval0, np.float64))
     mask = torch.from_numpy(free_mask(problem.K, problem.L, th0.shape[0]))
@@ -97,18 +98,21 @@ def lbfgs_recover(problem: RoomProblem, target_ir: np.ndarray, theta0: np.ndarra
     tgt = torch.from_numpy(np.asarray(target_ir, np.float64))
     opt = torch.optim.LBFGS([u, v], lr=lr, max_iter=1, history_size=20, line_search_fn="strong_wolfe")
     hist = []
+    first = []
 
     def closure():
         opt.zero_grad()
         try:
             ir = RoomIR.apply(th0 * torch.exp(u * mask), s0 * torch.exp(v), problem)
         except FasesError:
-            # unstable trial point: huge misfit, gradient pointing back to the origin
-            big = torch.tensor(1e30, dtype=torch.float64)
-            (big * ((u * mask) ** 2).sum() + big * (v ** 2).sum()).backward()
-            return big
+            # unstable trial point: a large finite misfit and no slope
+            u.grad = torch.zeros_like(u)
+            v.grad = torch.zeros_like(v)
+            return torch.tensor(1e3 * (first[0] if first else 1.0), dtype=torch.float64)
         J = ((ir - tgt) ** 2).sum()
         J.backward()
+        if not first:
+            first.append(float(J.detach()))
         return J
 
     for it in range(iterations):

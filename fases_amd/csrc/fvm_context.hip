@@ -212,6 +212,9 @@ int fases_create(const fases_params *p, fases_ctx **out) {
     if (p->precision != FASES_FP64 && p->precision != FASES_FP32 && p->precision != FASES_FP16)
         return set_error(nullptr, FASES_E_ARG, "fases_create: unknown precision");
     if (p->pml_cells < 1) return set_error(nullptr, FASES_E_ARG, "fases_create: pml_cells must be >= 1");
+    if (p->tile_rows != 0 && p->tile_rows != 8 && p->tile_rows != 16)
+        return set_error(nullptr, FASES_E_ARG, "fases_create: tile_rows must be 0, 8 or 16");
+    if (p->tile_z < 0) return set_error(nullptr, FASES_E_ARG, "fases_create: tile_z must be >= 0");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev < 1)
@@ -615,7 +618,7 @@ int fases_run(fases_ctx *ctx, int64_t nsteps, double *ir, double *vel, double *e
 int fases_tile_config(const fases_ctx *ctx, int *tile_x, int *tile_y, int *tile_z) {
     if (!ctx) return FASES_E_ARG;
     if (tile_x) *tile_x = tile::TXO;
-    if (tile_y) *tile_y = 16 - 2;
+    if (tile_y) *tile_y = ctx->nw - 2;
     if (tile_z) *tile_z = ctx->tz;
     return FASES_OK;
 }

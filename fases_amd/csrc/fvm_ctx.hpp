@@ -77,6 +77,7 @@ struct fases_ctx {
     int fexp[4] = {0, 0, 0, 0};
     unsigned *d_amax = nullptr;
     int tz = 0;                          // z planes per block of the tiled kernel (0: not tuned)
+    int nw = 16;                         // waves (rows) per block of the tiled kernel: 16 or 8
     void *adj = nullptr;                 // adjoint checkpoints and buffers (fvm_adjoint.hip)
     void (*adj_release)(void *) = nullptr;
 
@@ -193,12 +194,24 @@ bool uses_tiled(const fases_ctx *ctx) {
     return !(kv == 1 || (kv == 0 && ctx->prm.precision == FASES_FP64));
 }
 
-// Tile autotuner: times the stage-0 flux launch (set 0 -> set 1; set 1 is rewritten by
-// the next step's stage 0, and stage 0 leaves the PML states alone, so this changes no
-// state) for each candidate z depth, three times each, and keeps the fastest.
+// Tiled stage launch with the context's block height (16 waves: one block per CU and
+// less halo; 8 waves: two blocks per CU whose barriers do not wait on each other).
+template <typename S, typename R>
+int launch_tiled_nw(fases_ctx *ctx, int stage, const FieldSet<S> &fi, const FieldSet<S> &fb, const FieldSet<S> &fo,
+                    R rho, R c, R h, R dt, int nw, int tz) {
+    return nw == 8 ? launch_tiled<S, R, 8>(ctx, stage, fi, fb, fo, rho, c, h, dt, tz)
+                   : launch_tiled<S, R, 16>(ctx, stage, fi, fb, fo, rho, c, h, dt, tz);
+}
+
+// Tile autotuner: times a stage-0 launch (set 0 -> set 1) plus a stage-1 launch (sets
+// 0, 1 -> set 2) for each candidate block height and z depth, best of four, and keeps
+// the fastest. Sets 1 and 2 are rewritten by the next step before they are read and
+// neither stage advances the PML states, so tuning changes no state. A fixed tile_rows /
+// tile_z restricts the search to that value.
 template <typename S, typename R>
 int autotune_tile(fases_ctx *ctx) {
-    if (ctx->prm.tile_z > 0) { ctx->tz = ctx->prm.tile_z; return FASES_OK; }
+    const int fix_nw = ctx->prm.tile_rows, fix_tz = ctx->prm.tile_z;
+    if (fix_tz > 0 && fix_nw > 0) { ctx->tz = fix_tz; ctx->nw = fix_nw; return FASES_OK; }
     const fases_params &P = ctx->prm;
     const int nz = ctx->Nz - 6;
     const int cand[4] = {16, 32, 64, 128};
@@ -206,27 +219,35 @@ int autotune_tile(fases_ctx *ctx) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     float best = 1e30f;
-    int pick = 64;
-    for (int tz : cand) {
-        if (tz > 16 && (tz / 2) >= nz) break;   // deeper blocks than the grid add nothing
-        float tmin = 1e30f;
-        for (int rep = 0; rep < 3; ++rep) {
-            HIPCHK(hipEventRecord(e0, ctx->stream));
-            int rc = launch_tiled<S, R, 16>(ctx, 0, fset<S>(ctx, 0), fset<S>(ctx, 0), fset<S>(ctx, 1), (R)P.rho,
-                                            (R)P.c, (R)P.h, (R)P.dt, tz);
-            if (rc) return rc;
-            HIPCHK(hipEventRecord(e1, ctx->stream));
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            tmin = std::min(tmin, ms);
+    int pick = fix_tz > 0 ? fix_tz : 64, pick_nw = fix_nw > 0 ? fix_nw : 16;
+    for (int nw : {16, 8}) {
+        if (fix_nw > 0 && nw != fix_nw) continue;
+        for (int tz : cand) {
+            if (fix_tz > 0 && tz != fix_tz) continue;
+            if (fix_tz <= 0 && tz > 16 && (tz / 2) >= nz) break;   // deeper blocks than the grid add nothing
+            float tmin = 1e30f;
+            for (int rep = 0; rep < 4; ++rep) {
+                HIPCHK(hipEventRecord(e0, ctx->stream));
+                int rc = launch_tiled_nw<S, R>(ctx, 0, fset<S>(ctx, 0), fset<S>(ctx, 0), fset<S>(ctx, 1), (R)P.rho,
+                                               (R)P.c, (R)P.h, (R)P.dt, nw, tz);
+                if (rc) return rc;
+                rc = launch_tiled_nw<S, R>(ctx, 1, fset<S>(ctx, 1), fset<S>(ctx, 0), fset<S>(ctx, 2), (R)P.rho,
+                                           (R)P.c, (R)P.h, (R)P.dt, nw, tz);
+                if (rc) return rc;
+                HIPCHK(hipEventRecord(e1, ctx->stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                tmin = std::min(tmin, ms);
+            }
+            if (tmin < best) { best = tmin; pick = tz; pick_nw = nw; }
         }
-        if (tmin < best) { best = tmin; pick = tz; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HIPCHK(hipGetLastError());
     ctx->tz = pick;
+    ctx->nw = pick_nw;
     return FASES_OK;
 }
 
@@ -282,7 +303,7 @@ int launch_stages(fases_ctx *ctx, long long n, const FieldSet<S> sets[4], double
                 hipLaunchKernelGGL((k_flux_simple<S, R, 2>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml,
                                    rho, c, h, dt);
         } else {
-            rc = launch_tiled<S, R, 16>(ctx, s, sets[q.in], sets[0], sets[q.out], rho, c, h, dt, ctx->tz);
+            rc = launch_tiled_nw<S, R>(ctx, s, sets[q.in], sets[0], sets[q.out], rho, c, h, dt, ctx->nw, ctx->tz);
             if (rc) return rc;
         }
         if (kev) HIPCHK(hipEventRecord(e1, ctx->stream));

@@ -149,14 +149,21 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     const bool inner_row = (wy >= 1 && wy <= NW - 2);
     const bool out_lane = (lx >= 1 && lx <= TXO) && i < Nx - 3 && j < Ny - 3 && inner_row;
 
-    // halo item of this thread: p columns/rows, u columns, v rows
-    int h_kind = -1, h_r = 0, h_c = 0;
-    unsigned h_off = 0;
-    {
+    // halo items of this thread (p columns/rows, u columns, v rows): items tid, tid + 64 NW, ...
+    constexpr int NITEM = 4 * LY + 4 * 64 + 4 * NW + 4 * 64;
+    constexpr int NH = (NITEM + 64 * NW - 1) / (64 * NW);
+    bool h_on[NH];
+    const S *hsrc[NH];
+    R hsc[NH];
+    R *hdst[NH];
+    unsigned h_off[NH];
+#pragma unroll
+    for (int q = 0; q < NH; ++q) {
         const int nP1 = 4 * LY, nP2 = 4 * 64, nU = 4 * NW, nV = 4 * 64;
         const int sideX[4] = {0, 1, LX - 2, LX - 1};
         const int sideY[4] = {0, 1, LY - 2, LY - 1};
-        int t = tid;
+        int t = tid + q * 64 * NW;
+        int h_kind = -1, h_r = 0, h_c = 0;
         if (t < nP1) {
             h_kind = 0; h_c = sideX[t / LY]; h_r = t % LY;
         } else if ((t -= nP1) < nP2) {
@@ -166,17 +173,19 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         } else if ((t -= nU) < nV) {
             h_kind = 2; h_r = sideY[t / 64]; h_c = t % 64;
         }
+        h_on[q] = h_kind >= 0;
+        h_off[q] = 0;
         if (h_kind >= 0) {
             int gx = (h_kind == 2) ? (x0 - 1 + h_c) : (x0 - 3 + h_c);
             int gy = (h_kind == 1) ? (j0 - 1 + h_r) : (j0 - 3 + h_r);
             gx = max(0, min(gx, Nx - 1));
             gy = max(0, min(gy, Ny - 1));
-            h_off = (unsigned)(gx + Nx * gy);
+            h_off[q] = (unsigned)(gx + Nx * gy);
         }
+        hsrc[q] = (h_kind == 0) ? ip : (h_kind == 1 ? iu : iv);
+        hsc[q] = (h_kind == 0) ? sc.s[0] : (h_kind == 1 ? sc.s[1] : sc.s[2]);
+        hdst[q] = (h_kind == 0) ? &L.p[h_r][h_c] : (h_kind == 1 ? &L.u[h_r][h_c] : &L.v[h_r][h_c]);
     }
-    const S *hsrc = (h_kind == 0) ? ip : (h_kind == 1 ? iu : iv);
-    const R hsc = (h_kind == 0) ? sc.s[0] : (h_kind == 1 ? sc.s[1] : sc.s[2]);
-    R *hdst = (h_kind == 0) ? &L.p[h_r][h_c] : (h_kind == 1 ? &L.u[h_r][h_c] : &L.v[h_r][h_c]);
 
     // z pipeline: window p/w at k-1..k+3 (index 0..4), carried A-state of cell k and
     // flux through k-1/2
@@ -200,13 +209,15 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         for (int q = 0; q < 5; ++q) { pz[q] = p7[q + 2]; wz[q] = w7[q + 2]; }
     }
     // plane values in flight: own u, v (and p for the two outer rows), halo item
-    R nu, nv, np = 0, nh = 0;
+    R nu, nv, np = 0, nh[NH];
     {
         const long long pl = nxy * (long long)k0;
         nu = CV::ld(at(iu + pl, cb), sc.s[1]);
         nv = CV::ld(at(iv + pl, cb), sc.s[2]);
         if (!inner_row) np = CV::ld(at(ip + pl, cb), sc.s[0]);
-        if (h_kind >= 0) nh = CV::ld((hsrc + pl)[h_off], hsc);   // per-lane field: no uniform base
+#pragma unroll
+        for (int q = 0; q < NH; ++q)                       // per-lane field: no uniform base
+            nh[q] = h_on[q] ? CV::ld((hsrc[q] + pl)[h_off[q]], hsc[q]) : R(0);
     }
 
     for (int k = k0; k < kend; ++k) {
@@ -217,7 +228,9 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         L.p[wy + 2][lx + 2] = pc;
         L.u[wy][lx + 2] = uc;
         L.v[wy + 2][lx] = vc;
-        if (h_kind >= 0) *hdst = nh;
+#pragma unroll
+        for (int q = 0; q < NH; ++q)
+            if (h_on[q]) *hdst[q] = nh[q];
         __syncthreads();
         // ---- prefetch the next plane
         if (k + 1 < kend) {
@@ -225,7 +238,9 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
             nu = CV::ld(at(iu + pn, cb), sc.s[1]);
             nv = CV::ld(at(iv + pn, cb), sc.s[2]);
             if (!inner_row) np = CV::ld(at(ip + pn, cb), sc.s[0]);
-            if (h_kind >= 0) nh = CV::ld((hsrc + pn)[h_off], hsc);
+#pragma unroll
+            for (int q = 0; q < NH; ++q)
+                if (h_on[q]) nh[q] = CV::ld((hsrc[q] + pn)[h_off[q]], hsc[q]);
         }
         // ---- y states of every row (incl. the two outer rows)
         {

@@ -58,6 +58,7 @@ int fases_params_default(fases_params *p) {
     p->kernel_variant = 0;
     p->tile_z = 0;
     p->adjoint_variant = 0;
+    p->tile_rows = 0;
     return FASES_OK;
 }
 

@@ -255,6 +255,7 @@ class SolverParams:
     kernel_variant: int = 0             # 0 auto, 1 per-cell (reference order), 2 tiled
     tile_z: int = 0                     # tiled kernel z planes per block; 0 = autotune
     adjoint_variant: int = 0            # flux adjoint: 0 tiled, 1 per cell
+    tile_rows: int = 0                  # tiled kernel waves per block (8 or 16); 0 = autotune
 
     def to_c(self) -> _lib.fases_params:
         p = _lib.fases_params()
@@ -273,6 +274,7 @@ class SolverParams:
         p.kernel_variant = int(self.kernel_variant)
         p.tile_z = int(self.tile_z)
         p.adjoint_variant = int(self.adjoint_variant)
+        p.tile_rows = int(self.tile_rows)
         return p
 
 

@@ -75,6 +75,8 @@ typedef struct fases_params {
                                first fases_run (times 16, 32, 64, 128 on the real grid) */
     int    adjoint_variant; /* flux adjoint of fases_adjoint: 0 tiled (each WENO pair VJP
                                once), 1 per cell (neighbours' pairs re-evaluated)        */
+    int    tile_rows;       /* tiled kernel: waves (grid rows) per block, 8 or 16; 0 = autotune
+                               together with tile_z                                        */
 } fases_params;
 
 /* Fills *p with the reference defaults (FP64, device 0, PML depth 20, ...). */
