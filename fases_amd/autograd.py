@@ -81,47 +81,82 @@ def free_mask(K: int, L: int, n: int) -> np.ndarray:
 
 
 def lbfgs_recover(problem: RoomProblem, target_ir: np.ndarray, theta0: np.ndarray, sval0: np.ndarray,
-                  iterations: int = 50, lr: float = 0.02, warm: int = 3, log=None):
+                  iterations: int = 50, lr: float = 0.02, warm: int = 3, max_ls: int = 8, log=None):
     """Fits theta = theta0 * exp(u) (multiplicative, so signs and scales are kept; the pair
     frequencies stay fixed) and sval = sval0 * exp(v) to the target IRs by L-BFGS on
-    J = sum (ir - target)^2. Returns (theta, sval, history of J per iteration).
-    The line search starts at lr times the quasi-Newton step for the first `warm`
-    iterations (the first step is then at most lr per log-parameter: a full step can
-    push a wall past |R| = 1, where the room becomes unstable), at the full step
-    afterwards; a trial point whose run goes non-finite counts as 1e3 times the first
-    misfit with a zero gradient, so the line search backs off."""
+    J = sum (ir - target)^2. Returns (theta, sval, history) at the lowest misfit found;
+    history[0] is the starting misfit and history[n] the lowest after iteration n.
+    Each iteration is one quasi-Newton direction and a strong-Wolfe line search of at
+    most `max_ls` trial runs. The first trial is lr times the quasi-Newton step for the
+    first `warm` iterations (the first step is then at most lr per log-parameter: a full
+    step can push a wall past |R| = 1, where the room becomes unstable), the full step
+    afterwards; a trial point whose run fails, or whose misfit or gradient is not
+    finite, counts as 1e3 times the first misfit with a zero gradient, so the line
+    search brackets it and backs off."""
     th0 = torch.from_numpy(np.asarray(theta0, np.float64))
     s0 = torch.from_numpy(np.asarray(sval0, np.float64))
     mask = torch.from_numpy(free_mask(problem.K, problem.L, th0.shape[0]))
     u = torch.zeros_like(th0, requires_grad=True)
     v = torch.zeros_like(s0, requires_grad=True)
     tgt = torch.from_numpy(np.asarray(target_ir, np.float64))
-    opt = torch.optim.LBFGS([u, v], lr=lr, max_iter=1, history_size=20, line_search_fn="strong_wolfe")
-    hist = []
+    # one step() call runs the closure once, then the line search's first trial and at
+    # most max_eval - 1 more (torch's default max_eval for max_iter = 1 leaves the line
+    # search no trial beyond the first, so a failed trial could never be bracketed)
+    opt = torch.optim.LBFGS([u, v], lr=lr, max_iter=1, max_eval=max_ls, history_size=20,
+                            line_search_fn="strong_wolfe")
     first = []
+    runs = [0]
+    best = {"J": float("inf"), "u": u.detach().clone(), "v": v.detach().clone()}
+
+    def unstable():
+        # a large finite misfit and no slope (a NaN misfit would defeat the line
+        # search's comparisons and could be accepted)
+        u.grad = torch.zeros_like(u)
+        v.grad = torch.zeros_like(v)
+        return torch.tensor(1e3 * (first[0] if first else 1.0), dtype=torch.float64)
+
+    # every step() call first evaluates the closure at the current point, which the
+    # previous line search has already run: answer it from the evaluations kept here
+    seen = []
 
     def closure():
         opt.zero_grad()
+        for (cu, cv, cJ, gu, gv) in seen:
+            if torch.equal(cu, u.detach()) and torch.equal(cv, v.detach()):
+                u.grad = gu.clone()
+                v.grad = gv.clone()
+                return torch.tensor(cJ, dtype=torch.float64)
         try:
             ir = RoomIR.apply(th0 * torch.exp(u * mask), s0 * torch.exp(v), problem)
         except FasesError:
-            # unstable trial point: a large finite misfit and no slope
-            u.grad = torch.zeros_like(u)
-            v.grad = torch.zeros_like(v)
-            return torch.tensor(1e3 * (first[0] if first else 1.0), dtype=torch.float64)
+            return unstable()
         J = ((ir - tgt) ** 2).sum()
+        if not bool(torch.isfinite(J)):
+            # the run overflowed (FP32 storage does not stop on a non-finite value)
+            return unstable()
         J.backward()
+        if not (bool(torch.isfinite(u.grad).all()) and bool(torch.isfinite(v.grad).all())):
+            return unstable()
+        Jf = float(J.detach())
         if not first:
-            first.append(float(J.detach()))
+            first.append(Jf)
+        if Jf < best["J"]:
+            best.update(J=Jf, u=u.detach().clone(), v=v.detach().clone())
+        seen.append((u.detach().clone(), v.detach().clone(), Jf, u.grad.clone(), v.grad.clone()))
+        del seen[:-2 * max_ls]
+        runs[0] += 1
         return J
 
+    hist = []
     for it in range(iterations):
         if it == warm:
             for g in opt.param_groups:
                 g["lr"] = 1.0
         J = float(opt.step(closure).detach())
-        hist.append(J)
+        if it == 0:
+            hist.append(J)
+        hist.append(best["J"])
         if log:
-            log("L-BFGS iteration %d: J = %.6e" % (it, J))
+            log("L-BFGS iteration %d: J = %.6e (%d runs)" % (it, best["J"], runs[0]))
     with torch.no_grad():
-        return (th0 * torch.exp(u * mask)).numpy(), (s0 * torch.exp(v)).numpy(), hist
+        return (th0 * torch.exp(best["u"] * mask)).numpy(), (s0 * torch.exp(best["v"])).numpy(), hist

@@ -228,7 +228,7 @@ int build_phase(Batch &bt, const std::vector<fases_ctx *> &jobs, const std::vect
         FluxJob<S> f{};
         f.Nx = c->Nx; f.Ny = c->Ny; f.Nz = c->Nz; f.tz = c->tz;
         f.nbx = (c->Nx - 6 + tile::TXO - 1) / tile::TXO;
-        f.nby = (c->Ny - 6 + 13) / 14;
+        f.nby = (c->Ny - 6 + 15) / 16;
         const int nbz = (c->Nz - 6 + c->tz - 1) / c->tz;
         for (int s = 0; s < 3; ++s)
             for (int v = 0; v < 4; ++v) f.U[s][v] = (S *)c->U[s][v];

@@ -618,7 +618,7 @@ int fases_run(fases_ctx *ctx, int64_t nsteps, double *ir, double *vel, double *e
 int fases_tile_config(const fases_ctx *ctx, int *tile_x, int *tile_y, int *tile_z) {
     if (!ctx) return FASES_E_ARG;
     if (tile_x) *tile_x = tile::TXO;
-    if (tile_y) *tile_y = ctx->nw - 2;
+    if (tile_y) *tile_y = ctx->nw;
     if (tile_z) *tile_z = ctx->tz;
     return FASES_OK;
 }

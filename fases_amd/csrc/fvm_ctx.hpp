@@ -173,7 +173,7 @@ int launch_tiled(fases_ctx *ctx, int stage, const FieldSet<S> &fi, const FieldSe
     (void)rho; (void)c; (void)h;
     const TileK<R> K = tile_constants<R>(ctx->prm);
     const dim3 tb(64, NW);
-    const dim3 tg((ctx->Nx - 6 + tile::TXO - 1) / tile::TXO, (ctx->Ny - 6 + (NW - 2) - 1) / (NW - 2),
+    const dim3 tg((ctx->Nx - 6 + tile::TXO - 1) / tile::TXO, (ctx->Ny - 6 + NW - 1) / NW,
                   (ctx->Nz - 6 + tz - 1) / tz);
     const Scl<R> sc = field_scales<R>(ctx);
     const TileFields<S> F{fi.p, fi.u, fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w};

@@ -1,13 +1,15 @@
 // Tiled flux-stage kernel for gfx950: WENO5-Z + Godunov + SSP-RK3 stage + cPML with
 // every face reconstructed once.
 //
-// Block = 64 lanes (x columns x0-1 .. x0+62) x NW waves (y rows j0-1 .. j0+NW-2),
-// marching over TZ planes in z. Outputs are the inner 62 columns and NW-2 rows; the
-// outer lane / wave of each side only supplies face states to its neighbour.
+// Block = 64 lanes (x columns x0-1 .. x0+62) x NW waves (y rows j0 .. j0+NW-1),
+// marching over tz planes in z. Outputs are the inner 62 columns and all NW rows; the
+// outer lane of each side only supplies x-face states to its neighbour.
 //   z faces: register pipeline along the march (face k+1/2 computed once, carried);
 //   x faces: each lane reconstructs both states of its own cell, the right state of
 //            the next cell comes from the neighbouring lane (shuffle);
-//   y faces: both states per cell written to LDS, read by the rows above and below.
+//   y faces: both states per cell written to LDS, read by the rows above and below;
+//            the states of the two halo rows j0-1 and j0+NW (p and v) are four extra
+//            reconstructions, one each on waves 0, 1, NW-2, NW-1, i.e. one per SIMD.
 // p, u (x) and p, v (y) of the current plane sit in LDS with a 3-cell halo; p and w
 // along z stay in registers.
 //
@@ -16,6 +18,8 @@
 // differences, smoothness indicators 13/12 s^2 + 1/4 t^2 from second differences s and
 // the one-sided combinations t, weights d_r (1 + tau/(beta_r + eps)), normalised once.
 #pragma once
+
+#include <type_traits>
 
 #include "fvm_device.hpp"
 
@@ -73,10 +77,10 @@ constexpr int LX = 68;          // LDS columns: x0-3 .. x0+64
 }  // namespace tile
 
 template <typename R, int NW> struct TileLds {
-    R p[NW + 4][tile::LX];      // rows j0-3 .. j0+NW
+    R p[NW + 6][tile::LX];      // rows j0-3 .. j0+NW+2
     R u[NW][tile::LX];
-    R v[NW + 4][64];
-    R ys[4][NW][64];            // y states: p^-, p^+, v^-, v^+ of every row
+    R v[NW + 6][64];
+    R ys[4][NW + 2][64];        // y states p^-, p^+, v^-, v^+ of rows j0-1 .. j0+NW
 };
 
 // Constants of the tiled kernel as unevaluated sums hi + lo (lo = 0 for FP64): in a
@@ -108,13 +112,42 @@ __device__ inline R shfl_next(R x) { return __shfl_down(x, 1, 64); }   // value 
 template <typename R>
 __device__ inline R shfl_prev(R x) { return __shfl_up(x, 1, 64); }     // value of lane - 1
 
-// Element at a 32-bit byte offset from a base pointer (the offset of a lane within a
-// plane is computed once; the plane base changes per plane).
-template <typename T> __device__ inline T &at(T *base, unsigned boff) {
-    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff);
+// Plane-relative buffer access: a resource descriptor whose base is one z-plane of a
+// field (uniform, built in scalar registers) and the lane's 32-bit byte offset within
+// the plane, so a load or store costs no per-lane 64-bit address arithmetic. The
+// descriptor's range is the plane: the hardware returns 0 for an offset at or beyond
+// it and drops such a store. (Data-format word of the gfx9 family; stride 0 = raw.)
+__device__ inline __amdgpu_buffer_rsrc_t plane_rsrc(const void *plane, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(plane), (short)0, (int)bytes, 0x00020000);
 }
-template <typename T> __device__ inline const T &at(const T *base, unsigned boff) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+template <typename T> struct PlaneIO;
+template <> struct PlaneIO<float> {
+    __device__ static inline float ld(__amdgpu_buffer_rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    }
+    __device__ static inline void st(__amdgpu_buffer_rsrc_t r, unsigned off, float x) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)off, 0, 0);
+    }
+};
+template <> struct PlaneIO<double> {
+    __device__ static inline double ld(__amdgpu_buffer_rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    }
+    __device__ static inline void st(__amdgpu_buffer_rsrc_t r, unsigned off, double x) {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, (int)off, 0, 0);
+    }
+};
+template <> struct PlaneIO<__half> {
+    __device__ static inline __half ld(__amdgpu_buffer_rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(__half, __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0));
+    }
+    __device__ static inline void st(__amdgpu_buffer_rsrc_t r, unsigned off, __half x) {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, x), r, (int)off, 0, 0);
+    }
+};
+__device__ inline uint8_t plane_ld_u8(const uint8_t *plane, unsigned bytes, unsigned off) {
+    return __builtin_amdgcn_raw_buffer_load_b8(plane_rsrc(plane, bytes), (int)off, 0, 0);
 }
 
 // Field pointers of one tiled stage launch: input set, base set U0, output set.
@@ -124,14 +157,14 @@ template <typename S> struct TileFields {
     S *op, *ou, *ov, *ow;
 };
 
-// One tile = 64 x NW threads; outputs 62 columns x (NW-2) rows x tz planes of tile
+// One tile = 64 x NW threads; outputs 62 columns x NW rows x tz planes of tile
 // (bx, by, bz) of an Nx x Ny x Nz grid.
 template <typename S, typename R, int STAGE, int NW>
 __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, const TileFields<S> &F,
                                  const uint8_t *__restrict__ upd, const PmlSet &pml, const TileK<R> &K, R dt,
                                  const Scl<R> &sc, int tz, int bx, int by, int bz) {
     using namespace tile;
-    constexpr int LY = NW + 4, TYO = NW - 2;
+    constexpr int LY = NW + 6;
     typedef Conv<S, R> CV;
     const S *__restrict__ ip = F.ip, *__restrict__ iu = F.iu, *__restrict__ iv = F.iv, *__restrict__ iw = F.iw;
     const S *__restrict__ bp = F.bp, *__restrict__ bu = F.bu, *__restrict__ bv = F.bv, *__restrict__ bw = F.bw;
@@ -139,18 +172,28 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     // blockDim.x is one wavefront, so the row index is uniform per wave: keep it scalar
     const int lx = threadIdx.x, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int tid = wy * 64 + lx;
-    const int x0 = 3 + TXO * bx, j0 = 3 + TYO * by, k0 = 3 + tz * bz;
+    const int x0 = 3 + TXO * bx, j0 = 3 + NW * by, k0 = 3 + tz * bz;
     const long long nxy = (long long)Nx * Ny;
     const int kend = min(k0 + tz, Nz - 3);
-    const int i = x0 - 1 + lx, j = j0 - 1 + wy;
+    const int i = x0 - 1 + lx, j = j0 + wy;
     const int ic = min(i, Nx - 1), jc = min(j, Ny - 1);               // clamped for loads
     const unsigned col = (unsigned)(ic + Nx * jc);                   // offset within a plane
     const unsigned cb = col * (unsigned)sizeof(S);                   // ... in bytes
-    const bool inner_row = (wy >= 1 && wy <= NW - 2);
-    const bool out_lane = (lx >= 1 && lx <= TXO) && i < Nx - 3 && j < Ny - 3 && inner_row;
+    const unsigned pbytes = (unsigned)nxy * (unsigned)sizeof(S);     // bytes of one plane
+    const bool out_lane = (lx >= 1 && lx <= TXO) && i < Nx - 3 && j < Ny - 3;
+    // field value of this lane's column in the plane at element offset `pl`
+    auto LD = [&](const S *f, long long pl, R scale) __device__ {
+        return CV::ld(PlaneIO<S>::ld(plane_rsrc(f + pl, pbytes), cb), scale);
+    };
+    auto ST = [&](S *f, long long pl, R x, R inv_scale) __device__ {
+        PlaneIO<S>::st(plane_rsrc(f + pl, pbytes), cb, CV::st(x, inv_scale));
+    };
 
-    // halo items of this thread (p columns/rows, u columns, v rows): items tid, tid + 64 NW, ...
-    constexpr int NITEM = 4 * LY + 4 * 64 + 4 * NW + 4 * 64;
+    // halo items of this thread: items tid, tid + 64 NW, ... of
+    //   p: rows 0-2 and NW+3..NW+5 (all LX columns), columns 0, 1, LX-2, LX-1 of the own rows;
+    //   u: columns 0, 1, LX-2, LX-1 of the own rows;  v: rows 0-2 and NW+3..NW+5
+    constexpr int nP1 = 6 * LX, nP2 = 4 * NW, nU = 4 * NW, nV = 6 * 64;
+    constexpr int NITEM = nP1 + nP2 + nU + nV;
     constexpr int NH = (NITEM + 64 * NW - 1) / (64 * NW);
     bool h_on[NH];
     const S *hsrc[NH];
@@ -159,15 +202,14 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     unsigned h_off[NH];
 #pragma unroll
     for (int q = 0; q < NH; ++q) {
-        const int nP1 = 4 * LY, nP2 = 4 * 64, nU = 4 * NW, nV = 4 * 64;
         const int sideX[4] = {0, 1, LX - 2, LX - 1};
-        const int sideY[4] = {0, 1, LY - 2, LY - 1};
+        const int sideY[6] = {0, 1, 2, NW + 3, NW + 4, NW + 5};
         int t = tid + q * 64 * NW;
         int h_kind = -1, h_r = 0, h_c = 0;
         if (t < nP1) {
-            h_kind = 0; h_c = sideX[t / LY]; h_r = t % LY;
+            h_kind = 0; h_r = sideY[t / LX]; h_c = t % LX;
         } else if ((t -= nP1) < nP2) {
-            h_kind = 0; h_r = sideY[t / 64]; h_c = 2 + t % 64;
+            h_kind = 0; h_c = sideX[t / NW]; h_r = 3 + t % NW;
         } else if ((t -= nP2) < nU) {
             h_kind = 1; h_c = sideX[t / NW]; h_r = t % NW;
         } else if ((t -= nU) < nV) {
@@ -177,7 +219,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         h_off[q] = 0;
         if (h_kind >= 0) {
             int gx = (h_kind == 2) ? (x0 - 1 + h_c) : (x0 - 3 + h_c);
-            int gy = (h_kind == 1) ? (j0 - 1 + h_r) : (j0 - 3 + h_r);
+            int gy = (h_kind == 1) ? (j0 + h_r) : (j0 - 3 + h_r);
             gx = max(0, min(gx, Nx - 1));
             gy = max(0, min(gy, Ny - 1));
             h_off[q] = (unsigned)(gx + Nx * gy);
@@ -187,17 +229,16 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         hdst[q] = (h_kind == 0) ? &L.p[h_r][h_c] : (h_kind == 1 ? &L.u[h_r][h_c] : &L.v[h_r][h_c]);
     }
 
-    // z pipeline: window p/w at k-1..k+3 (index 0..4), carried A-state of cell k and
-    // flux through k-1/2
-    R pz[5] = {0, 0, 0, 0, 0}, wz[5] = {0, 0, 0, 0, 0};
-    R pA = 0, wA = 0, FzP = 0, FzW = 0;
-    if (inner_row) {
+    // z pipeline: window p/w at k-1..k+3, carried A-state of cell k and flux through k-1/2
+    R pz[5], wz[5];
+    R pA, wA, FzP, FzW;
+    {
         R p7[7], w7[7];
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
             const long long pq = nxy * (long long)(k0 - 3 + q);
-            p7[q] = CV::ld(at(ip + pq, cb), sc.s[0]);
-            w7[q] = CV::ld(at(iw + pq, cb), sc.s[3]);
+            p7[q] = LD(ip, pq, sc.s[0]);
+            w7[q] = LD(iw, pq, sc.s[3]);
         }
         R pAm, wAm, pB0, wB0, tmp0, tmp1;
         weno_pair(p7[0], p7[1], p7[2], p7[3], p7[4], pAm, tmp0);   // cell k0-1
@@ -208,55 +249,72 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
 #pragma unroll
         for (int q = 0; q < 5; ++q) { pz[q] = p7[q + 2]; wz[q] = w7[q + 2]; }
     }
-    // plane values in flight: own u, v (and p for the two outer rows), halo item
-    R nu, nv, np = 0, nh[NH];
+    // plane values in flight: own u, v, halo items
+    R nu, nv, nh[NH];
     {
         const long long pl = nxy * (long long)k0;
-        nu = CV::ld(at(iu + pl, cb), sc.s[1]);
-        nv = CV::ld(at(iv + pl, cb), sc.s[2]);
-        if (!inner_row) np = CV::ld(at(ip + pl, cb), sc.s[0]);
+        nu = LD(iu, pl, sc.s[1]);
+        nv = LD(iv, pl, sc.s[2]);
 #pragma unroll
         for (int q = 0; q < NH; ++q)                       // per-lane field: no uniform base
             nh[q] = h_on[q] ? CV::ld((hsrc[q] + pl)[h_off[q]], hsc[q]) : R(0);
     }
 
-    for (int k = k0; k < kend; ++k) {
+    // One plane k. PH = (k - k0) mod 5 is a compile-time constant: logical window slot q
+    // (plane k - 1 + q) lives in register pz[(q + PH) % 5].
+    auto plane = [&](int k, auto phase) __device__ __attribute__((always_inline)) {
+        constexpr int PH = decltype(phase)::value;
         const long long pl = nxy * (long long)k;
         // ---- fill the plane tiles from the values loaded during the previous plane
-        const R pc = inner_row ? pz[1] : np;
-        const R uc = nu, vc = nv;
-        L.p[wy + 2][lx + 2] = pc;
+        const R pc = pz[(1 + PH) % 5], uc = nu, vc = nv, wc = wz[(1 + PH) % 5];
+        L.p[wy + 3][lx + 2] = pc;
         L.u[wy][lx + 2] = uc;
-        L.v[wy + 2][lx] = vc;
+        L.v[wy + 3][lx] = vc;
 #pragma unroll
         for (int q = 0; q < NH; ++q)
             if (h_on[q]) *hdst[q] = nh[q];
         __syncthreads();
         // ---- prefetch the next plane
+        R pnew = 0, wnew = 0;
         if (k + 1 < kend) {
             const long long pn = pl + nxy;
-            nu = CV::ld(at(iu + pn, cb), sc.s[1]);
-            nv = CV::ld(at(iv + pn, cb), sc.s[2]);
-            if (!inner_row) np = CV::ld(at(ip + pn, cb), sc.s[0]);
+            nu = LD(iu, pn, sc.s[1]);
+            nv = LD(iv, pn, sc.s[2]);
 #pragma unroll
             for (int q = 0; q < NH; ++q)
                 if (h_on[q]) nh[q] = CV::ld((hsrc[q] + pn)[h_off[q]], hsc[q]);
+            // z window: plane k+4
+            const long long p4 = nxy * (long long)(k + 4);
+            pnew = LD(ip, p4, sc.s[0]);
+            wnew = LD(iw, p4, sc.s[3]);
         }
-        // ---- y states of every row (incl. the two outer rows)
+        // ---- y states of the own row, and of the halo rows on four waves
         {
             R A, B, Av, Bv;
-            weno_pair(L.p[wy][lx + 2], L.p[wy + 1][lx + 2], pc, L.p[wy + 3][lx + 2], L.p[wy + 4][lx + 2], A, B);
-            weno_pair(L.v[wy][lx], L.v[wy + 1][lx], vc, L.v[wy + 3][lx], L.v[wy + 4][lx], Av, Bv);
-            L.ys[0][wy][lx] = A;
-            L.ys[1][wy][lx] = B;
-            L.ys[2][wy][lx] = Av;
-            L.ys[3][wy][lx] = Bv;
+            weno_pair(L.p[wy + 1][lx + 2], L.p[wy + 2][lx + 2], pc, L.p[wy + 4][lx + 2], L.p[wy + 5][lx + 2], A, B);
+            weno_pair(L.v[wy + 1][lx], L.v[wy + 2][lx], vc, L.v[wy + 4][lx], L.v[wy + 5][lx], Av, Bv);
+            L.ys[0][wy + 1][lx] = A;
+            L.ys[1][wy + 1][lx] = B;
+            L.ys[2][wy + 1][lx] = Av;
+            L.ys[3][wy + 1][lx] = Bv;
+            if (wy == 0 || wy == NW - 2) {           // p of row j0-1 (wave 0) / j0+NW (wave NW-2)
+                const int r = wy == 0 ? 0 : NW + 1;  // ys row; its p window is L.p rows r..r+4
+                weno_pair(L.p[r][lx + 2], L.p[r + 1][lx + 2], L.p[r + 2][lx + 2], L.p[r + 3][lx + 2],
+                          L.p[r + 4][lx + 2], A, B);
+                L.ys[0][r][lx] = A;
+                L.ys[1][r][lx] = B;
+            } else if (wy == 1 || wy == NW - 1) {    // v of row j0-1 (wave 1) / j0+NW (wave NW-1)
+                const int r = wy == 1 ? 0 : NW + 1;
+                weno_pair(L.v[r][lx], L.v[r + 1][lx], L.v[r + 2][lx], L.v[r + 3][lx], L.v[r + 4][lx], Av, Bv);
+                L.ys[2][r][lx] = Av;
+                L.ys[3][r][lx] = Bv;
+            }
         }
-        // ---- x faces (shuffles are wave-wide: every lane of an inner row takes part)
-        R FxP_p = 0, FxP_u = 0, FxM_p = 0, FxM_u = 0;
-        if (inner_row) {
+        // ---- x faces (shuffles are wave-wide: every lane takes part)
+        R FxP_p, FxP_u, FxM_p, FxM_u;
+        {
             R A, B, Au, Bu;
-            weno_pair(L.p[wy + 2][lx], L.p[wy + 2][lx + 1], pc, L.p[wy + 2][lx + 3], L.p[wy + 2][lx + 4], A, B);
+            weno_pair(L.p[wy + 3][lx], L.p[wy + 3][lx + 1], pc, L.p[wy + 3][lx + 3], L.p[wy + 3][lx + 4], A, B);
             weno_pair(L.u[wy][lx], L.u[wy][lx + 1], uc, L.u[wy][lx + 3], L.u[wy][lx + 4], Au, Bu);
             const R Bn = shfl_next(B), Bun = shfl_next(Bu);
             godunov_f(A, Bn, Au, Bun, K, FxP_p, FxP_u);          // face i+1/2
@@ -265,66 +323,72 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         }
         __syncthreads();
 
-        R pAn = 0, wAn = 0, FzP_p = 0, FzP_w = 0, pnew = 0, wnew = 0;
-        if (inner_row) {
-            // ---- y faces j+1/2 and j-1/2
-            R FyP_p, FyP_v, FyM_p, FyM_v;
-            godunov_f(L.ys[0][wy][lx], L.ys[1][wy + 1][lx], L.ys[2][wy][lx], L.ys[3][wy + 1][lx], K, FyP_p, FyP_v);
-            godunov_f(L.ys[0][wy - 1][lx], L.ys[1][wy][lx], L.ys[2][wy - 1][lx], L.ys[3][wy][lx], K, FyM_p, FyM_v);
-            // ---- z face k+1/2 from the new cell k+1
-            R pBn, wBn;
-            weno_pair(pz[0], pz[1], pz[2], pz[3], pz[4], pAn, pBn);
-            weno_pair(wz[0], wz[1], wz[2], wz[3], wz[4], wAn, wBn);
-            godunov_f(pA, pBn, wA, wBn, K, FzP_p, FzP_w);
-            const R wc = wz[1];
-            // z window for the next plane: issue the lead loads before the update
-            if (k + 1 < kend) {
-                const long long p4 = nxy * (long long)(k + 4);
-                pnew = CV::ld(at(ip + p4, cb), sc.s[0]);
-                wnew = CV::ld(at(iw + p4, cb), sc.s[3]);
-            }
-            if (out_lane && at(upd + pl, col) == 1) {
-                const R sp = -FxP_p + FxM_p - FyP_p + FyM_p - FzP_p + FzP;
-                const R Du = -FxP_u + FxM_u, Dv = -FyP_v + FyM_v, Dw = -FzP_w + FzW;
-                R uo, vo, wo, po, wgt;
-                if (STAGE == 0) {
-                    uo = fma_s(K.dth, Du, uc);
-                    vo = fma_s(K.dth, Dv, vc);
-                    wo = fma_s(K.dth, Dw, wc);
-                    po = fma_s(K.dth, sp, pc);
-                    wgt = R(1);
+        // ---- y faces j+1/2 and j-1/2
+        R FyP_p, FyP_v, FyM_p, FyM_v;
+        godunov_f(L.ys[0][wy + 1][lx], L.ys[1][wy + 2][lx], L.ys[2][wy + 1][lx], L.ys[3][wy + 2][lx], K, FyP_p,
+                  FyP_v);
+        godunov_f(L.ys[0][wy][lx], L.ys[1][wy + 1][lx], L.ys[2][wy][lx], L.ys[3][wy + 1][lx], K, FyM_p, FyM_v);
+        // ---- z face k+1/2 from the new cell k+1
+        R pAn, wAn, pBn, wBn, FzP_p, FzP_w;
+        weno_pair(pz[(0 + PH) % 5], pz[(1 + PH) % 5], pz[(2 + PH) % 5], pz[(3 + PH) % 5], pz[(4 + PH) % 5], pAn,
+                  pBn);
+        weno_pair(wz[(0 + PH) % 5], wz[(1 + PH) % 5], wz[(2 + PH) % 5], wz[(3 + PH) % 5], wz[(4 + PH) % 5], wAn,
+                  wBn);
+        godunov_f(pA, pBn, wA, wBn, K, FzP_p, FzP_w);
+        if (out_lane && plane_ld_u8(upd + pl, (unsigned)nxy, col) == 1) {
+            const R sp = -FxP_p + FxM_p - FyP_p + FyM_p - FzP_p + FzP;
+            const R Du = -FxP_u + FxM_u, Dv = -FyP_v + FyM_v, Dw = -FzP_w + FzW;
+            R uo, vo, wo, po, wgt;
+            if (STAGE == 0) {
+                uo = fma_s(K.dth, Du, uc);
+                vo = fma_s(K.dth, Dv, vc);
+                wo = fma_s(K.dth, Dw, wc);
+                po = fma_s(K.dth, sp, pc);
+                wgt = R(1);
+            } else {
+                const R p0 = LD(bp, pl, sc.s[0]), u0 = LD(bu, pl, sc.s[1]);
+                const R v0 = LD(bv, pl, sc.s[2]), w0 = LD(bw, pl, sc.s[3]);
+                if (STAGE == 1) {
+                    uo = fma(R(0.75), u0, R(0.25) * fma_s(K.dth, Du, uc));
+                    vo = fma(R(0.75), v0, R(0.25) * fma_s(K.dth, Dv, vc));
+                    wo = fma(R(0.75), w0, R(0.25) * fma_s(K.dth, Dw, wc));
+                    po = fma(R(0.75), p0, R(0.25) * fma_s(K.dth, sp, pc));
+                    wgt = R(0.25);
                 } else {
-                    const R p0 = CV::ld(at(bp + pl, cb), sc.s[0]), u0 = CV::ld(at(bu + pl, cb), sc.s[1]);
-                    const R v0 = CV::ld(at(bv + pl, cb), sc.s[2]), w0 = CV::ld(at(bw + pl, cb), sc.s[3]);
-                    if (STAGE == 1) {
-                        uo = fma(R(0.75), u0, R(0.25) * fma_s(K.dth, Du, uc));
-                        vo = fma(R(0.75), v0, R(0.25) * fma_s(K.dth, Dv, vc));
-                        wo = fma(R(0.75), w0, R(0.25) * fma_s(K.dth, Dw, wc));
-                        po = fma(R(0.75), p0, R(0.25) * fma_s(K.dth, sp, pc));
-                        wgt = R(0.25);
-                    } else {
-                        uo = mul_s(K.third, fma(R(2), fma_s(K.dth, Du, uc), u0));
-                        vo = mul_s(K.third, fma(R(2), fma_s(K.dth, Dv, vc), v0));
-                        wo = mul_s(K.third, fma(R(2), fma_s(K.dth, Dw, wc), w0));
-                        po = mul_s(K.third, fma(R(2), fma_s(K.dth, sp, pc), p0));
-                        wgt = R(2) / R(3);
-                    }
+                    uo = mul_s(K.third, fma(R(2), fma_s(K.dth, Du, uc), u0));
+                    vo = mul_s(K.third, fma(R(2), fma_s(K.dth, Dv, vc), v0));
+                    wo = mul_s(K.third, fma(R(2), fma_s(K.dth, Dw, wc), w0));
+                    po = mul_s(K.third, fma(R(2), fma_s(K.dth, sp, pc), p0));
+                    wgt = R(2) / R(3);
                 }
-                if (pml.on)
-                    pml_apply<R, STAGE == 2>(pml, i, j, k, Nx, Ny, po, uo, vo, wo, (-FxP_p + FxM_p) * K.ih,
-                                             (-FyP_p + FyM_p) * K.ih, (-FzP_p + FzP) * K.ih, Du * K.ih, Dv * K.ih,
-                                             Dw * K.ih, wgt * dt);
-                at(ou + pl, cb) = CV::st(uo, sc.is[1]);
-                at(ov + pl, cb) = CV::st(vo, sc.is[2]);
-                at(ow + pl, cb) = CV::st(wo, sc.is[3]);
-                at(op + pl, cb) = CV::st(po, sc.is[0]);
             }
+            if (pml.on)
+                pml_apply<R, STAGE == 2>(pml, i, j, k, Nx, Ny, po, uo, vo, wo, (-FxP_p + FxM_p) * K.ih,
+                                         (-FyP_p + FyM_p) * K.ih, (-FzP_p + FzP) * K.ih, Du * K.ih, Dv * K.ih,
+                                         Dw * K.ih, wgt * dt);
+            ST(ou, pl, uo, sc.is[1]);
+            ST(ov, pl, vo, sc.is[2]);
+            ST(ow, pl, wo, sc.is[3]);
+            ST(op, pl, po, sc.is[0]);
         }
-        // advance the z pipeline (every row: the outer rows carry zeros, so the window
-        // registers are the same on both sides of the row branch and nothing is copied)
+        // advance the z pipeline: the window is a ring indexed at compile time, so the
+        // advance moves no registers (slot 0, plane k-1, becomes slot 4, plane k+4)
         pA = pAn; wA = wAn; FzP = FzP_p; FzW = FzP_w;
-        pz[0] = pz[1]; pz[1] = pz[2]; pz[2] = pz[3]; pz[3] = pz[4]; pz[4] = pnew;
-        wz[0] = wz[1]; wz[1] = wz[2]; wz[2] = wz[3]; wz[3] = wz[4]; wz[4] = wnew;
+        pz[PH] = pnew;
+        wz[PH] = wnew;
+    };
+    // five planes per trip so the ring returns to its start; kend is uniform, so every
+    // wave leaves at the same plane and the barriers stay matched
+    for (int k = k0; k < kend; k += 5) {
+        plane(k, std::integral_constant<int, 0>{});
+        if (k + 1 >= kend) break;
+        plane(k + 1, std::integral_constant<int, 1>{});
+        if (k + 2 >= kend) break;
+        plane(k + 2, std::integral_constant<int, 2>{});
+        if (k + 3 >= kend) break;
+        plane(k + 3, std::integral_constant<int, 3>{});
+        if (k + 4 >= kend) break;
+        plane(k + 4, std::integral_constant<int, 4>{});
     }
 }
 

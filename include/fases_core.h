@@ -226,12 +226,12 @@ int fases_adjoint(fases_ctx *ctx, const double *dJ_dir, double *grad_mat, double
  * common/gui.py:11137-11168, each stepping main3D.cu:1280-1320 alone). */
 int fases_batch_run(fases_ctx **ctx, int n, const int64_t *nsteps, double **ir, double *elapsed_ms);
 
-/* Sizes: padded cells, boundary cells, room (updated) cells, device bytes in use. */
 /* Tile of the tiled flux kernel in use: output columns x rows x z planes per block
    (*tile_z = 0 before the first fases_run when autotuning). No counterpart in the
    reference (one thread per cell there); reported so runs can state their launch shape. */
 int fases_tile_config(const fases_ctx *ctx, int *tile_x, int *tile_y, int *tile_z);
 
+/* Sizes: padded cells, boundary cells, room (updated) cells, device bytes in use. */
 int fases_info(const fases_ctx *ctx, int64_t *n_cells, int64_t *n_bnd, int64_t *n_upd,
                int64_t *dev_bytes);
 
