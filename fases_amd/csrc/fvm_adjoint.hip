@@ -140,10 +140,13 @@ int build_probe_csr(fases_ctx *ctx, AdjBufs &A) {
     if ((rc = adj_alloc(ctx, A, &A.toff, off.size()))) return rc;
     if ((rc = adj_alloc(ctx, A, &A.eb, eb.size()))) return rc;
     if ((rc = adj_alloc(ctx, A, &A.ew, ew.size()))) return rc;
-    HIPCHK(hipMemcpy(A.tgt, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(A.toff, off.data(), off.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(A.eb, eb.data(), eb.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(A.ew, ew.data(), ew.size() * 8, hipMemcpyHostToDevice));
+    // on the context stream, behind the zero fill adj_alloc queued there (the stream is
+    // non-blocking: a null-stream copy could land first and then be zeroed)
+    HIPCHK(hipMemcpyAsync(A.tgt, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(A.toff, off.data(), off.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(A.eb, eb.data(), eb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(A.ew, ew.data(), ew.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));          // the host vectors end here
     return FASES_OK;
 }
 
@@ -225,8 +228,10 @@ int setup(fases_ctx *ctx, AdjBufs &A, long long nsteps, int every) {
         A.n_ap = (int)apc.size();
         if ((rc = adj_alloc(ctx, A, &A.ap_cells, apc.size()))) return rc;
         if ((rc = adj_alloc(ctx, A, &A.ap_idx, api.size()))) return rc;
-        HIPCHK(hipMemcpy(A.ap_cells, apc.data(), apc.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(A.ap_idx, api.data(), api.size() * 4, hipMemcpyHostToDevice));
+        // ordered after adj_alloc's zero fill on the (non-blocking) context stream
+        HIPCHK(hipMemcpyAsync(A.ap_cells, apc.data(), apc.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(A.ap_idx, api.data(), api.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));      // apc / api end with this block
         for (auto &l : A.lam_sh)
             if ((rc = adj_alloc(ctx, A, &l, 4 * (size_t)std::max(A.n_ap, 1)))) return rc;
     }

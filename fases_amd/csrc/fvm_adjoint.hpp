@@ -41,25 +41,29 @@ __device__ inline void weno_pair_vjp(R a, R b, R c, R d, R e, R mA, R mB, R g[5]
     const R dl = fma(R(0.3), g0, fma(R(0.6), g1, R(0.1) * g2));
     const R dr = fma(R(0.1), g0, fma(R(0.6), g1, R(0.3) * g2));
     const R rl = fast_rcp(dl), rr = fast_rcp(dr);
-    // A = c + nl rl, B = c + nr rr
-    const R mnl = mA * rl, mnr = mB * rr;
-    const R mdl = -mnl * (nl * rl), mdr = -mnr * (nr * rr);
-    // g_r enter nl, nr, dl, dr
-    const R mg0 = fma(mnl, a0, fma(mnr, b0, fma(R(0.3), mdl, R(0.1) * mdr)));
-    const R mg1 = fma(mnl, a1, fma(mnr, b1, R(0.6) * (mdl + mdr)));
-    const R mg2 = fma(mnl, a2, fma(mnr, b2, fma(R(0.1), mdl, R(0.3) * mdr)));
-    // candidate offsets (linear in the differences)
-    const R ma0 = mnl * g0, ma1 = mnl * g1, ma2 = mnl * g2;
-    const R mb0 = mnr * g0, mb1 = mnr * g1, mb2 = mnr * g2;
+    // A = c + sum_r W_r a'_r with normalised weights W_r = d_r g_r / dl (a_r = d_r a'_r):
+    // every quantity below stays bounded by the data and 1 / eps. (Expanding through
+    // nl / dl^2 and the products of e instead forms terms ~ mA / dl ~ mA / e^3 that cancel
+    // to the result: at the eps level that overflows and loses the result to rounding.)
+    const R Ao = nl * rl, Bo = nr * rr;                      // A - c, B - c
+    const R uA0 = g0 * rl, uA1 = g1 * rl, uA2 = g2 * rl;     // dA/da_r = W_r / d_r
+    const R uB0 = g0 * rr, uB1 = g1 * rr, uB2 = g2 * rr;
+    const R ma0 = mA * uA0, ma1 = mA * uA1, ma2 = mA * uA2;
+    const R mb0 = mB * uB0, mb1 = mB * uB1, mb2 = mB * uB2;
     R md1 = fma(R(-1.0 / 30.0), ma2, R(0.05) * mb2);
     R md2 = fma(R(0.1), ma1, fma(R(1.0 / 12.0), ma2, R(-0.2) * (mb1 + mb2)));
     R md3 = fma(R(0.2), ma0 + ma1, fma(R(-1.0 / 12.0), mb0, R(-0.1) * mb1));
     R md4 = fma(R(-0.05), ma0, R(1.0 / 30.0) * mb0);
-    // g_r = (e_r + tau) prod_{s != r} e_s
-    const R mtau = fma(mg0, p12, fma(mg1, p02, mg2 * p01));
-    R me0 = fma(mg0, p12, fma(mg1, (e1 + tau) * e2, mg2 * ((e2 + tau) * e1)));
-    const R me1 = fma(mg0, (e0 + tau) * e2, fma(mg1, p02, mg2 * ((e2 + tau) * e0)));
-    R me2 = fma(mg0, (e0 + tau) * e1, fma(mg1, (e1 + tau) * e0, mg2 * p01));
+    // W_r = d_r (1 + tau/e_r) / sum: dA/de_s = -(a'_s - Ao) W_s tau / (e_s (e_s + tau)),
+    // dA/dtau = sum_r (a'_r - Ao) W_r / (e_r + tau); Q_r = mA W_r (a'_r - Ao) + (B side)
+    const R Q0 = fma(ma0, fma(R(-0.3), Ao, a0), mb0 * fma(R(-0.1), Bo, b0));
+    const R Q1 = fma(ma1, fma(R(-0.6), Ao, a1), mb1 * fma(R(-0.6), Bo, b1));
+    const R Q2 = fma(ma2, fma(R(-0.1), Ao, a2), mb2 * fma(R(-0.3), Bo, b2));
+    const R R0 = fast_rcp(e0 * (e0 + tau)), R1 = fast_rcp(e1 * (e1 + tau)), R2 = fast_rcp(e2 * (e2 + tau));
+    const R mtau = fma(Q0, e0 * R0, fma(Q1, e1 * R1, Q2 * (e2 * R2)));
+    R me0 = -Q0 * (tau * R0);
+    const R me1 = -Q1 * (tau * R1);
+    R me2 = -Q2 * (tau * R2);
     const R sg = df > R(0) ? R(1) : (df < R(0) ? R(-1) : R(0));
     me0 = fma(sg, mtau, me0);
     me2 = fma(-sg, mtau, me2);

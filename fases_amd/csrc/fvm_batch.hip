@@ -206,6 +206,7 @@ struct Batch {
         HIPCHK(hipMalloc(&p, std::max<size_t>(1, v.size()) * sizeof(T)));
         allocs.push_back(p);
         if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));        // callers pass short-lived host tables
         *out = static_cast<T *>(p);
         return FASES_OK;
     }

@@ -348,6 +348,7 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
             if ((r = upload(ctx, &dbb, bb.data(), bb.size()))) return r;
             if ((r = upload(ctx, &da, aa.data(), aa.size()))) return r;
             ax.band = db; ax.kc = dk; ax.b = dbb; ax.a = da;
+            HIPCHK(hipStreamSynchronize(ctx->stream));   // the next axis refills the host tables
             return FASES_OK;
         };
         if ((rc = axis(ctx->pml.x, Nx, key[3], key[2]))) return rc;   // L = x-, R = x+
@@ -466,6 +467,7 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
             for (size_t q = 0; q < patch.size(); ++q) pidx[(size_t)patch[q]] = (int)q;
             HIPCHK(hipMemcpyAsync(ctx->d_pidx, pidx.data(), pidx.size() * sizeof(int), hipMemcpyHostToDevice,
                                   ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));   // pidx ends with this block
             if ((rc = upload(ctx, &ctx->d_patch, patch.data(), patch.size()))) return rc;
             for (int q = 0; q < 3; ++q) {
                 if ((rc = dalloc(ctx, &ctx->shadow[q], (size_t)4 * patch.size()))) return rc;
