@@ -84,7 +84,8 @@ def lbfgs_recover(problem: RoomProblem, target_ir: np.ndarray, theta0: np.ndarra
                   iterations: int = 50, lr: float = 0.02, warm: int = 3, max_ls: int = 8, log=None):
     """Fits theta = theta0 * exp(u) (multiplicative, so signs and scales are kept; the pair
     frequencies stay fixed) and sval = sval0 * exp(v) to the target IRs by L-BFGS on
-    J = sum (ir - target)^2. Returns (theta, sval, history) at the lowest misfit found;
+    J = sum (ir - target)^2 (normalised by sum target^2 inside the optimiser). Returns
+    (theta, sval, history) at the lowest misfit found;
     history[0] is the starting misfit and history[n] the lowest after iteration n.
     Each iteration is one quasi-Newton direction and a strong-Wolfe line search of at
     most `max_ls` trial runs. The first trial is lr times the quasi-Newton step for the
@@ -99,11 +100,16 @@ def lbfgs_recover(problem: RoomProblem, target_ir: np.ndarray, theta0: np.ndarra
     u = torch.zeros_like(th0, requires_grad=True)
     v = torch.zeros_like(s0, requires_grad=True)
     tgt = torch.from_numpy(np.asarray(target_ir, np.float64))
+    # the optimiser sees J / sum(target^2): torch's L-BFGS has absolute thresholds (the
+    # curvature pair is kept only if y.s > 1e-10), so the misfit has to be O(1)-scaled
+    jscale = float((tgt ** 2).sum()) or 1.0
     # one step() call runs the closure once, then the line search's first trial and at
     # most max_eval - 1 more (torch's default max_eval for max_iter = 1 leaves the line
     # search no trial beyond the first, so a failed trial could never be bracketed)
+    # torch's absolute stopping tolerances (gradient 1e-7, change 1e-9) would end the run
+    # early on a small misfit; the iteration count is the caller's, so they are off
     opt = torch.optim.LBFGS([u, v], lr=lr, max_iter=1, max_eval=max_ls, history_size=20,
-                            line_search_fn="strong_wolfe")
+                            tolerance_grad=0.0, tolerance_change=0.0, line_search_fn="strong_wolfe")
     first = []
     runs = [0]
     best = {"J": float("inf"), "u": u.detach().clone(), "v": v.detach().clone()}
@@ -130,7 +136,7 @@ def lbfgs_recover(problem: RoomProblem, target_ir: np.ndarray, theta0: np.ndarra
             ir = RoomIR.apply(th0 * torch.exp(u * mask), s0 * torch.exp(v), problem)
         except FasesError:
             return unstable()
-        J = ((ir - tgt) ** 2).sum()
+        J = ((ir - tgt) ** 2).sum() / jscale
         if not bool(torch.isfinite(J)):
             # the run overflowed (FP32 storage does not stop on a non-finite value)
             return unstable()
@@ -154,9 +160,9 @@ def lbfgs_recover(problem: RoomProblem, target_ir: np.ndarray, theta0: np.ndarra
                 g["lr"] = 1.0
         J = float(opt.step(closure).detach())
         if it == 0:
-            hist.append(J)
-        hist.append(best["J"])
+            hist.append(J * jscale)
+        hist.append(best["J"] * jscale)
         if log:
-            log("L-BFGS iteration %d: J = %.6e (%d runs)" % (it, best["J"], runs[0]))
+            log("L-BFGS iteration %d: J = %.6e (%d runs)" % (it, best["J"] * jscale, runs[0]))
     with torch.no_grad():
         return (th0 * torch.exp(best["u"] * mask)).numpy(), (s0 * torch.exp(best["v"])).numpy(), hist

@@ -91,3 +91,20 @@ def test_lbfgs_does_not_rerun_the_accepted_point():
     # the first point once, then the trials: on a quadratic the unit step is accepted
     # at the first trial (two runs per iteration if the accepted point were run again)
     assert p.calls <= 1 + 10 + 3
+
+
+def test_lbfgs_iterates_on_a_tiny_misfit():
+    """The misfit scale is the caller's (Pa^2 sums of IR differences can be ~1e-10): the
+    driver must take the same steps whatever the scale, and not stop on torch's absolute
+    gradient / change / curvature thresholds."""
+    th0 = np.ones((6, 1 + 2 * LinearRoom.K + 4 * LinearRoom.L))
+    s0 = np.ones(1)
+    hists = []
+    for scale in (1.0, 1e-6):
+        p = LinearRoom("none")
+        p.M *= scale
+        tgt = p.forward(th0 * 1.05, s0 * 0.97)
+        th, sv, hist = AG.lbfgs_recover(p, tgt, th0, s0, iterations=15)
+        hists.append(np.array(hist) / scale ** 2)
+    assert hists[0][-1] < 0.01 * hists[0][0]
+    np.testing.assert_allclose(hists[1], hists[0], rtol=1e-6)
