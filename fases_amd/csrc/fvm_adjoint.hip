@@ -50,6 +50,7 @@ struct AdjBufs {
     // adjoint patch: the forward patch cells within kAdjointPatch of a source
     int n_ap = 0;
     int *ap_cells = nullptr, *ap_idx = nullptr;
+    double *ap_part = nullptr;                   // pair-VJP partials of the patch, [15][2][n_ap]
     AdeState lx[4] = {}, rx1 = {}, rx2 = {};
     double *lD = nullptr, *G = nullptr, *gs = nullptr, *gus = nullptr, *dj = nullptr;
     long long gus_n = 0;
@@ -234,6 +235,7 @@ int setup(fases_ctx *ctx, AdjBufs &A, long long nsteps, int every) {
         HIPCHK(hipStreamSynchronize(ctx->stream));      // apc / api end with this block
         for (auto &l : A.lam_sh)
             if ((rc = adj_alloc(ctx, A, &l, 4 * (size_t)std::max(A.n_ap, 1)))) return rc;
+        if ((rc = adj_alloc(ctx, A, &A.ap_part, 30 * (size_t)std::max(A.n_ap, 1)))) return rc;
     }
     for (auto &x : A.lx)
         if ((rc = alloc_ade(ctx, A, x))) return rc;
@@ -405,15 +407,15 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
                 if (ctx->nB > 0) {
                     if (s == 2)
                         hipLaunchKernelGGL((k_boundary_adj<S, R, true, true>), bgrid, dim3(sb), 0, ctx->stream, g, t.vin,
-                                           bd, ade, P.rho, P.c, P.dt, t.a, t.b, t.c, X0, *t.xr, A.lx[t.lnew],
+                                           bd, ade, P.rho, P.c, P.dt, t.a, t.b, 1.0 / t.c, X0, *t.xr, A.lx[t.lnew],
                                            A.lx[ix0], A.lx[t.lr], Lo.p, Lo.u, Lo.v, Lo.w, A.lD, A.G, A.P, scd);
                     else if (s == 1)
                         hipLaunchKernelGGL((k_boundary_adj<S, R, false, true>), bgrid, dim3(sb), 0, ctx->stream, g,
-                                           t.vin, bd, ade, P.rho, P.c, P.dt, t.a, t.b, t.c, X0, *t.xr, A.lx[t.lnew],
+                                           t.vin, bd, ade, P.rho, P.c, P.dt, t.a, t.b, 1.0 / t.c, X0, *t.xr, A.lx[t.lnew],
                                            A.lx[ix0], A.lx[t.lr], Lo.p, Lo.u, Lo.v, Lo.w, A.lD, A.G, A.P, scd);
                     else
                         hipLaunchKernelGGL((k_boundary_adj<S, R, false, false>), bgrid, dim3(sb), 0, ctx->stream, g,
-                                           t.vin, bd, ade, P.rho, P.c, P.dt, t.a, t.b, t.c, X0, *t.xr, A.lx[t.lnew],
+                                           t.vin, bd, ade, P.rho, P.c, P.dt, t.a, t.b, 1.0 / t.c, X0, *t.xr, A.lx[t.lnew],
                                            A.lx[ix0], A.lx[t.lr], Lo.p, Lo.u, Lo.v, Lo.w, A.lD, A.G, A.P, scd);
                 }
                 if (ctx->n_src > 0)
@@ -451,21 +453,26 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
                     const AdjK<double> Kd{0.5 * P.rho * P.c * P.c, 0.5 * P.c, 0.5 / P.rho, t.beta * P.dt / P.h,
                                           t.alpha, t.beta};
                     const dim3 pg((unsigned)((nap + 63) / 64));
+                    const dim3 qg((unsigned)((15LL * nap + 63) / 64));
+                    const PatchAdjIn<S> pin{t.vin, t.vsh, ctx->d_pidx, (int)np, scd};
+                    hipLaunchKernelGGL((k_flux_adj_patch_part<S, R>), qg, dim3(64), 0, ctx->stream, g, pin, Lo.p,
+                                       Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], ctx->d_upd, A.ap_cells, A.ap_idx, nap, Kd,
+                                       A.ap_part);
                     if (s == 2)
                         hipLaunchKernelGGL((k_flux_adj_patch<S, R, false, false, true>), pg, dim3(64), 0, ctx->stream,
-                                           g, t.vin, t.vsh, ctx->d_pidx, (int)np, scd, Lo.p, Lo.u, Lo.v, Lo.w,
-                                           A.lam_sh[t.lout], ctx->d_upd, A.ap_cells, A.ap_idx, nap, Li.p, Li.u, Li.v,
-                                           Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w, A.lam_sh[iU0], Kd);
+                                           Lo.p, Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], A.ap_cells, A.ap_idx, nap,
+                                           A.ap_part, Li.p, Li.u, Li.v, Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w,
+                                           A.lam_sh[iU0], Kd);
                     else if (s == 1)
                         hipLaunchKernelGGL((k_flux_adj_patch<S, R, false, true, true>), pg, dim3(64), 0, ctx->stream,
-                                           g, t.vin, t.vsh, ctx->d_pidx, (int)np, scd, Lo.p, Lo.u, Lo.v, Lo.w,
-                                           A.lam_sh[t.lout], ctx->d_upd, A.ap_cells, A.ap_idx, nap, Li.p, Li.u, Li.v,
-                                           Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w, A.lam_sh[iU0], Kd);
+                                           Lo.p, Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], A.ap_cells, A.ap_idx, nap,
+                                           A.ap_part, Li.p, Li.u, Li.v, Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w,
+                                           A.lam_sh[iU0], Kd);
                     else
                         hipLaunchKernelGGL((k_flux_adj_patch<S, R, true, false, false>), pg, dim3(64), 0, ctx->stream,
-                                           g, t.vin, t.vsh, ctx->d_pidx, (int)np, scd, Lo.p, Lo.u, Lo.v, Lo.w,
-                                           A.lam_sh[t.lout], ctx->d_upd, A.ap_cells, A.ap_idx, nap, Li.p, Li.u, Li.v,
-                                           Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w, A.lam_sh[iU0], Kd);
+                                           Lo.p, Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], A.ap_cells, A.ap_idx, nap,
+                                           A.ap_part, Li.p, Li.u, Li.v, Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w,
+                                           A.lam_sh[iU0], Kd);
                 }
                 if (ctx->nB > 0)
                     hipLaunchKernelGGL((k_probe_gather<R>), tgrid, dim3(sb), 0, ctx->stream, A.nT, A.tgt, A.toff, A.eb,
@@ -486,7 +493,7 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
         std::fill(grad_mat, grad_mat + (size_t)ctx->nMat * A.P, 0.0);
         for (int b = 0; b < ctx->nB; ++b) {
             double *o = grad_mat + (size_t)(mat[(size_t)b] & 0xFF) * A.P;
-            for (int p = 0; p < A.P; ++p) o[p] += Gh[(size_t)b * A.P + p];
+            for (int p = 0; p < A.P; ++p) o[p] += Gh[(size_t)p * ctx->nB + b];
         }
     }
     if (grad_sval && ctx->n_src > 0) HIPCHK(hipMemcpy(grad_sval, A.gs, (size_t)ctx->n_src * 8, hipMemcpyDeviceToHost));

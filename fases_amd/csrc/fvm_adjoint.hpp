@@ -176,34 +176,89 @@ k_flux_adj_cell(Grid g, FieldSet<S> V, Scl<R> sc, const R *__restrict__ L0, cons
 // states from the stage input's shadow (forward patch map vidx) and the output adjoint
 // from its FP64 adjoint shadow (adjoint patch map pidx; stored copies elsewhere); writes
 // the FP64 adjoint shadows of the input (and base) set and their stored copies, after
-// the grid kernel has run.
+// the grid kernel has run. Two launches: k_flux_adj_patch_part evaluates, for every patch
+// cell C and every (axis, q) of its 15 window positions, the pair VJPs of the cell
+// C - q s at C's slot (one thread each, 15 np threads: the per-cell sum alone leaves the
+// GPU nearly empty); k_flux_adj_patch adds them up in the per-cell order (bit-identical
+// to a single pass) and writes the results.
+template <typename S>
+struct PatchAdjIn {
+    FieldSet<S> V;                 // stage input (stored)
+    const double *vsh;             // its FP64 shadows, [4][nv]
+    const int *vidx;               // grid -> forward patch index
+    int nv;
+    Scl<double> sc;
+};
+
+template <typename S, typename R>
+__global__ void __launch_bounds__(64)
+k_flux_adj_patch_part(Grid g, PatchAdjIn<S> in, const R *__restrict__ L0, const R *__restrict__ L1,
+                      const R *__restrict__ L2, const R *__restrict__ L3, const double *__restrict__ lsh_out,
+                      const uint8_t *__restrict__ upd, const int *__restrict__ cells, const int *__restrict__ pidx,
+                      int np, AdjK<double> K, double *__restrict__ part) {
+    const long long tt = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (tt >= 15LL * np) return;
+    const int t = (int)(tt % np), combo = (int)(tt / np);     // combo = 5 ax + (q + 2)
+    const int ax = combo / 5, q = combo % 5 - 2;
+    const int C = cells[t];
+    const int pos[3] = {C % g.Nx, (C / g.Nx) % g.Ny, (int)(C / g.nxy)};
+    const int nax[3] = {g.Nx, g.Ny, g.Nz};
+    const long long s = ax == 0 ? 1LL : (ax == 1 ? (long long)g.Nx : g.nxy);
+    const int fu = 1 + ax;
+    const R *Lf[4] = {L0, L1, L2, L3};
+    const S *Vf[4] = {in.V.p, in.V.u, in.V.v, in.V.w};
+    auto lraw = [&](int f, long long x) -> double {
+        const int w = pidx[x];
+        return w >= 0 ? lsh_out[(long long)f * np + w] : (double)Lf[f][x];
+    };
+    auto lt = [&](int f, long long x) -> double { return upd[x] ? K.bdh * lraw(f, x) : 0.0; };
+    auto vv = [&](int f, long long x) -> double {
+        const int w = in.vidx[x];
+        return w >= 0 ? in.vsh[(long long)f * in.nv + w] : Conv<S, double>::ld(Vf[f][x], in.sc.s[f]);
+    };
+    double cp = 0.0, cu = 0.0;
+    const int cpos = pos[ax] - q;
+    if (cpos >= 2 && cpos <= nax[ax] - 3) {
+        const long long Cq = C - q * s;
+        const double lp0 = lt(0, Cq), lu0 = lt(fu, Cq);
+        const double mFpP = lt(0, Cq + s) - lp0, mFuP = lt(fu, Cq + s) - lu0;
+        const double mFpM = lp0 - lt(0, Cq - s), mFuM = lu0 - lt(fu, Cq - s);
+        double mpL, mpR, muL, muR, tpL, tpR, tuL, tuR;
+        godunov_vjp(mFpP, mFuP, K.kp, K.kc, K.kr, mpL, tpR, muL, tuR);
+        godunov_vjp(mFpM, mFuM, K.kp, K.kc, K.kr, tpL, mpR, tuL, muR);
+        double gp[5], gu[5];
+        weno_pair_vjp(vv(0, Cq - 2 * s), vv(0, Cq - s), vv(0, Cq), vv(0, Cq + s), vv(0, Cq + 2 * s), mpL, mpR, gp);
+        weno_pair_vjp(vv(fu, Cq - 2 * s), vv(fu, Cq - s), vv(fu, Cq), vv(fu, Cq + s), vv(fu, Cq + 2 * s), muL, muR,
+                      gu);
+        cp = gp[q + 2];
+        cu = gu[q + 2];
+    }
+    part[(2LL * combo) * np + t] = cp;
+    part[(2LL * combo + 1) * np + t] = cu;
+}
+
 template <typename S, typename R, bool ACC_IN, bool ACC_BASE, bool HAS_BASE>
 __global__ void __launch_bounds__(64)
-k_flux_adj_patch(Grid g, FieldSet<S> V, const double *__restrict__ vsh, const int *__restrict__ vidx, int nv,
-                 Scl<double> sc, const R *__restrict__ L0, const R *__restrict__ L1, const R *__restrict__ L2,
-                 const R *__restrict__ L3, const double *__restrict__ lsh_out, const uint8_t *__restrict__ upd,
-                 const int *__restrict__ cells, const int *__restrict__ pidx, int np, R *lin0, R *lin1, R *lin2,
+k_flux_adj_patch(const R *__restrict__ L0, const R *__restrict__ L1, const R *__restrict__ L2,
+                 const R *__restrict__ L3, const double *__restrict__ lsh_out, const int *__restrict__ cells,
+                 const int *__restrict__ pidx, int np, const double *__restrict__ part, R *lin0, R *lin1, R *lin2,
                  R *lin3, double *lsh_in, R *lb0, R *lb1, R *lb2, R *lb3, double *lsh_base, AdjK<double> K) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= np) return;
     const int C = cells[t];
-    const int i = C % g.Nx, j = (C / g.Nx) % g.Ny, k = (int)(C / g.nxy);
     const R *Lf[4] = {L0, L1, L2, L3};
-    const S *Vf[4] = {V.p, V.u, V.v, V.w};
-    auto lraw = [&](int f, long long q) -> double {
-        const int w = pidx[q];
-        return w >= 0 ? lsh_out[(long long)f * np + w] : (double)Lf[f][q];
-    };
-    auto lt = [&](int f, long long q) -> double { return upd[q] ? K.bdh * lraw(f, q) : 0.0; };
-    // forward values: FP64 shadows of the forward patch (index vidx, nv cells)
-    auto vv = [&](int f, long long q) -> double {
-        const int w = vidx[q];
-        return w >= 0 ? vsh[(long long)f * nv + w] : Conv<S, double>::ld(Vf[f][q], sc.s[f]);
+    auto lraw = [&](int f, long long x) -> double {
+        const int w = pidx[x];
+        return w >= 0 ? lsh_out[(long long)f * np + w] : (double)Lf[f][x];
     };
     double acc[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) acc[f] = K.beta * lraw(f, C);
-    flux_adj_sum<double>(g, i, j, k, vv, lt, K, acc);
+#pragma unroll
+    for (int combo = 0; combo < 15; ++combo) {       // ax-major, q = -2..2: the per-cell order
+        acc[0] += part[(2LL * combo) * np + t];
+        acc[1 + combo / 5] += part[(2LL * combo + 1) * np + t];
+    }
     R *li[4] = {lin0, lin1, lin2, lin3};
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
@@ -533,12 +588,13 @@ k_ade_replay(Grid g, FieldSet<S> V0, FieldSet<S> V1, Bnd<double> bd, Ade ade, do
 // adding, so the buffers need no clearing).
 // Lout (adjoint of the stage output) is read and replaced by (1 - chi) Lout on the
 // boundary cell; lD[4 b + f] receives the adjoint of the probe values (p, u, v, w);
-// G[P b + .] accumulates the material-parameter gradient of the cell, layout
-// (Rinf, A[K], zeta[K], B[L], C[L], alpha[L], beta[L]).
+// G[p nB + b] accumulates the material-parameter gradient of the cell, p over
+// (Rinf, A[K], zeta[K], B[L], C[L], alpha[L], beta[L]) (entry-major, so the accumulation
+// is coalesced). The stage coefficients come as a, b and 1/c.
 template <typename S, typename R, bool SET_MID, bool SET_R>
 __global__ void __launch_bounds__(256)
 k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd<double> bd, Ade ade, double rho, double c, double dt, double as,
-               double bs, double cs, AdeState xmid, AdeState xr, AdeState lnew, AdeState lmid, AdeState lr,
+               double bs, double ic, AdeState xmid, AdeState xr, AdeState lnew, AdeState lmid, AdeState lr,
                R *L0, R *L1, R *L2, R *L3, double *__restrict__ lD, double *__restrict__ G, int P,
                Scl<double> sc) {
 #pragma clang fp contract(off)
@@ -550,7 +606,8 @@ k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd<double> bd, Ade ade, double rho, dou
     const double Z0 = rho * c;
     const double wout = probe_wout(Vin, g, base, mw, q[0], q[1], q[2], nx, ny, nz, Z0, sc);
     const int K = ade.K, L = ade.L;
-    double *Gb = G + (long long)P * b;
+    const long long nB = bd.n;
+    double *Gb = G + b;                                 // entry p at Gb[p nB]
     // output adjoint of the cell, then its pre-blend share
     const double lp = (double)L0[C], lu = (double)L1[C], lv = (double)L2[C], lw = (double)L3[C];
     L0[C] = (R)((1.0 - chi) * lp);
@@ -564,41 +621,38 @@ k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd<double> bd, Ade ade, double rho, dou
     const double Rinf = ade.Rinf[m];
     lwout += Rinf * lwin;
     Gb[0] += lwin * wout;
-    const double f = bs * dt / cs;
+    const double ac = as * ic, bc = bs * ic, f = bc * dt;    // a/c, b/c, b dt/c
     for (int kk = 0; kk < ade.Keff[m]; ++kk) {
         const long long s = (long long)b * K + kk;
         const int mi = m * K + kk;
         const double z = ade.zeta[mi];
         const double x_r = xr.phi[s], x_m = xmid.phi[s];
-        double xn = as * x_m + bs * (x_r + dt * (wout - z * x_r));
-        xn /= cs;
+        const double xn = (as * x_m + bs * (x_r + dt * (wout - z * x_r))) * ic;
         const double ln = lnew.phi[s] + ade.A[mi] * lwin;
-        Gb[1 + kk] += lwin * xn;
+        Gb[(1 + kk) * nB] += lwin * xn;
         // stage 0 has mid == r: both terms land in one buffer, mid first
-        lmid.phi[s] = (SET_MID ? 0.0 : lmid.phi[s]) + (as / cs) * ln;
-        lr.phi[s] = (SET_R ? 0.0 : lr.phi[s]) + (bs / cs) * (1.0 - dt * z) * ln;
+        lmid.phi[s] = (SET_MID ? 0.0 : lmid.phi[s]) + ac * ln;
+        lr.phi[s] = (SET_R ? 0.0 : lr.phi[s]) + bc * (1.0 - dt * z) * ln;
         lwout += f * ln;
-        Gb[1 + K + kk] += -f * x_r * ln;
+        Gb[(1 + K + kk) * nB] += -f * x_r * ln;
     }
     for (int l = 0; l < ade.Leff[m]; ++l) {
         const long long s = (long long)b * L + l;
         const int mi = m * L + l;
         const double al = ade.alpha[mi], be = ade.beta[mi];
         const double y1 = xr.p1[s], y2 = xr.p2[s];
-        double y1n = as * xmid.p1[s] + bs * (y1 + dt * (wout - (al * y1 + be * y2)));
-        y1n /= cs;
-        double y2n = as * xmid.p2[s] + bs * (y2 + dt * (be * y1 - al * y2));
-        y2n /= cs;
+        const double y1n = (as * xmid.p1[s] + bs * (y1 + dt * (wout - (al * y1 + be * y2)))) * ic;
+        const double y2n = (as * xmid.p2[s] + bs * (y2 + dt * (be * y1 - al * y2))) * ic;
         const double l1 = lnew.p1[s] + ade.B[mi] * lwin, l2 = lnew.p2[s] + ade.C[mi] * lwin;
-        Gb[1 + 2 * K + l] += lwin * y1n;
-        Gb[1 + 2 * K + L + l] += lwin * y2n;
-        lmid.p1[s] = (SET_MID ? 0.0 : lmid.p1[s]) + (as / cs) * l1;
-        lmid.p2[s] = (SET_MID ? 0.0 : lmid.p2[s]) + (as / cs) * l2;
-        lr.p1[s] = (SET_R ? 0.0 : lr.p1[s]) + (bs / cs) * (l1 * (1.0 - dt * al) + l2 * (dt * be));
-        lr.p2[s] = (SET_R ? 0.0 : lr.p2[s]) + (bs / cs) * (l1 * (-dt * be) + l2 * (1.0 - dt * al));
+        Gb[(1 + 2 * K + l) * nB] += lwin * y1n;
+        Gb[(1 + 2 * K + L + l) * nB] += lwin * y2n;
+        lmid.p1[s] = (SET_MID ? 0.0 : lmid.p1[s]) + ac * l1;
+        lmid.p2[s] = (SET_MID ? 0.0 : lmid.p2[s]) + ac * l2;
+        lr.p1[s] = (SET_R ? 0.0 : lr.p1[s]) + bc * (l1 * (1.0 - dt * al) + l2 * (dt * be));
+        lr.p2[s] = (SET_R ? 0.0 : lr.p2[s]) + bc * (l1 * (-dt * be) + l2 * (1.0 - dt * al));
         lwout += f * l1;
-        Gb[1 + 2 * K + 2 * L + l] += f * (-y1 * l1 - y2 * l2);
-        Gb[1 + 2 * K + 3 * L + l] += f * (-y2 * l1 + y1 * l2);
+        Gb[(1 + 2 * K + 2 * L + l) * nB] += f * (-y1 * l1 - y2 * l2);
+        Gb[(1 + 2 * K + 3 * L + l) * nB] += f * (-y2 * l1 + y1 * l2);
     }
     // wout = pD + Z0 un, tangential part of the probe velocity kept
     const double lun = Z0 * lwout - lunb;

@@ -37,10 +37,14 @@ template <typename S> struct FluxJob {
     Scl<float> sc;
 };
 
+// Rows (waves) per block of the batched flux kernel: 8-wave blocks, two per CU, overlap
+// each other's barriers (the faster shape on the 512x384x256 room and on the batch rooms).
+constexpr int kBatchRows = 8;
+
 template <typename S, int STAGE>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(64 * kBatchRows)
 k_flux_batch(const FluxJob<S> *__restrict__ jobs, const int *__restrict__ boff, int nj, int parity) {
-    __shared__ TileLds<float, 16> L;
+    __shared__ TileLds<float, kBatchRows> L;
     const int j = job_of(boff, nj, blockIdx.x);
     const FluxJob<S> &J = jobs[j];
     const int lb = blockIdx.x - boff[j];
@@ -52,7 +56,7 @@ k_flux_batch(const FluxJob<S> *__restrict__ jobs, const int *__restrict__ boff, 
     const PmlSet pml = J.pml;
     const TileK<float> K = J.K;
     const Scl<float> sc = J.sc;
-    flux_tile<S, float, STAGE, 16>(L, J.Nx, J.Ny, J.Nz, F, J.upd, pml, K, J.dt, sc, J.tz, bx, by, bz);
+    flux_tile<S, float, STAGE, kBatchRows>(L, J.Nx, J.Ny, J.Nz, F, J.upd, pml, K, J.dt, sc, J.tz, bx, by, bz);
 }
 
 template <typename S> struct PatchJob {
@@ -229,7 +233,7 @@ int build_phase(Batch &bt, const std::vector<fases_ctx *> &jobs, const std::vect
         FluxJob<S> f{};
         f.Nx = c->Nx; f.Ny = c->Ny; f.Nz = c->Nz; f.tz = c->tz;
         f.nbx = (c->Nx - 6 + tile::TXO - 1) / tile::TXO;
-        f.nby = (c->Ny - 6 + 15) / 16;
+        f.nby = (c->Ny - 6 + kBatchRows - 1) / kBatchRows;
         const int nbz = (c->Nz - 6 + c->tz - 1) / c->tz;
         for (int s = 0; s < 3; ++s)
             for (int v = 0; v < 4; ++v) f.U[s][v] = (S *)c->U[s][v];
@@ -298,11 +302,11 @@ void launch_batch_step(hipStream_t st, const Phase<S> &ph, int np_jobs, int ns_j
     for (int s = 0; s < 3; ++s) {
         if (ph.nblk > 0) {
             if (s == 0)
-                hipLaunchKernelGGL((k_flux_batch<S, 0>), dim3(ph.nblk), dim3(64, 16), 0, st, ph.flux, ph.boff, ph.nj, parity);
+                hipLaunchKernelGGL((k_flux_batch<S, 0>), dim3(ph.nblk), dim3(64, kBatchRows), 0, st, ph.flux, ph.boff, ph.nj, parity);
             else if (s == 1)
-                hipLaunchKernelGGL((k_flux_batch<S, 1>), dim3(ph.nblk), dim3(64, 16), 0, st, ph.flux, ph.boff, ph.nj, parity);
+                hipLaunchKernelGGL((k_flux_batch<S, 1>), dim3(ph.nblk), dim3(64, kBatchRows), 0, st, ph.flux, ph.boff, ph.nj, parity);
             else
-                hipLaunchKernelGGL((k_flux_batch<S, 2>), dim3(ph.nblk), dim3(64, 16), 0, st, ph.flux, ph.boff, ph.nj, parity);
+                hipLaunchKernelGGL((k_flux_batch<S, 2>), dim3(ph.nblk), dim3(64, kBatchRows), 0, st, ph.flux, ph.boff, ph.nj, parity);
         }
         if (ph.npatch > 0) {
             const dim3 g((ph.npatch + 63) / 64);
