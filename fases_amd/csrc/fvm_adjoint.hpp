@@ -364,49 +364,64 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
         }
     }
 
-    // z windows of the own column: V p, w at k-2 .. k+2, scaled output adjoint at k-1 .. k+1
-    R zp[5], zw[5], ap[3], aw[3];
-    R za[5] = {0, 0, 0, 0, 0}, zb[5] = {0, 0, 0, 0, 0};        // z contributions to planes k-2 .. k+2 (p, w)
-    R xa[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};            // x/y contributions, planes k-2 .. k (p, u, v)
+    // Plane-relative access of the own column (buffer descriptors: scalar plane base,
+    // 32-bit lane offset; planes clamped to the grid like the halo items).
+    const unsigned cS = (unsigned)col * (unsigned)sizeof(S), cR = (unsigned)col * (unsigned)sizeof(R);
+    const unsigned pS = (unsigned)nxy * (unsigned)sizeof(S), pR = (unsigned)nxy * (unsigned)sizeof(R);
+    auto vld = [&](const S *f, int kk, R scale) __device__ -> R {
+        return CV::ld(PlaneIO<S>::ld(plane_rsrc(f + plane(kk), pS), cS), scale);
+    };
+    auto rld = [&](const R *a, long long pl) __device__ -> R { return PlaneIO<R>::ld(plane_rsrc(a + pl, pR), cR); };
+    auto rst = [&](R *a, long long pl, R x) __device__ { PlaneIO<R>::st(plane_rsrc(a + pl, pR), cR, x); };
+    auto uld = [&](long long pl) __device__ -> unsigned { return plane_ld_u8(upd + pl, (unsigned)nxy, (unsigned)col); };
+    auto ltp = [&](const R *a, long long pl) __device__ -> R { return uld(pl) ? bdh * rld(a, pl) : R(0); };
+
+    // Rings of five slots indexed at compile time (slot of logical entry q at phase PH:
+    // (q + PH) % 5), so advancing a plane moves no registers:
+    //   zp, zw: V p, w of the own column at planes k-2 .. k+2;
+    //   ap, aw: scaled output adjoint at k-1 .. k+1 (three live slots);
+    //   za, zb: z contributions to planes k-2 .. k+2 (p, w);
+    //   xa:     x/y contributions to planes k-2 .. k (p, u, v; three live slots).
+    R zp[5], zw[5], ap[5], aw[5];
+    R za[5] = {0, 0, 0, 0, 0}, zb[5] = {0, 0, 0, 0, 0};
+    R xa[5][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     {
         const int kb = k0 - 2;
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-            const long long pq = plane(kb - 2 + q) + col;
-            zp[q] = CV::ld(V.p[pq], sc.s[0]);
-            zw[q] = CV::ld(V.w[pq], sc.s[3]);
+            zp[q] = vld(V.p, kb - 2 + q, sc.s[0]);
+            zw[q] = vld(V.w, kb - 2 + q, sc.s[3]);
         }
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const int kq = kb - 1 + q;
-            const long long pq = plane(kq) + col;
             const bool inside = kq >= 0 && kq < Nz;
-            ap[q] = inside ? lt(L0, pq) : R(0);
-            aw[q] = inside ? lt(L3, pq) : R(0);
+            ap[q] = inside ? ltp(L0, nxy * (long long)kq) : R(0);
+            aw[q] = inside ? ltp(L3, nxy * (long long)kq) : R(0);
         }
+        ap[3] = ap[4] = aw[3] = aw[4] = R(0);
     }
-    for (int k = k0 - 2; k <= kend + 1; ++k) {
+    auto step = [&](int k, auto phase) __device__ __attribute__((always_inline)) {
+        constexpr int PH = decltype(phase)::value;
+        constexpr int Z0 = PH % 5, Z1 = (PH + 1) % 5, Z2 = (PH + 2) % 5, Z3 = (PH + 3) % 5, Z4 = (PH + 4) % 5;
         const bool xy = (k >= k0 && k < kend);                 // block-uniform
         // lead loads for the next plane's windows
-        R np_ = 0, nw_ = 0, nap = 0, naw = 0;
-        {
-            const long long p3 = plane(k + 3) + col;
-            np_ = CV::ld(V.p[p3], sc.s[0]);
-            nw_ = CV::ld(V.w[p3], sc.s[3]);
-            if (k + 2 < Nz) {
-                const long long p2 = nxy * (long long)(k + 2) + col;
-                nap = lt(L0, p2);
-                naw = lt(L3, p2);
-            }
+        const R np_ = vld(V.p, k + 3, sc.s[0]);
+        const R nw_ = vld(V.w, k + 3, sc.s[3]);
+        R nap = 0, naw = 0;
+        if (k + 2 < Nz) {
+            const long long p2 = nxy * (long long)(k + 2);
+            nap = ltp(L0, p2);
+            naw = ltp(L3, p2);
         }
         if (xy) {
             const long long pk = nxy * (long long)k;
-            T.vp[wy + 2][lx + 2] = zp[2];
-            T.vu[wy][lx + 2] = CV::ld(V.u[pk + col], sc.s[1]);
-            T.vv[wy + 2][lx] = CV::ld(V.v[pk + col], sc.s[2]);
-            T.lp[wy + 1][lx + 1] = ap[1];
-            T.lu[wy][lx + 1] = lt(L1, pk + col);
-            T.lv[wy + 1][lx] = lt(L2, pk + col);
+            T.vp[wy + 2][lx + 2] = zp[Z2];
+            T.vu[wy][lx + 2] = vld(V.u, k, sc.s[1]);
+            T.vv[wy + 2][lx] = vld(V.v, k, sc.s[2]);
+            T.lp[wy + 1][lx + 1] = ap[Z1];
+            T.lu[wy][lx + 1] = ltp(L1, pk);
+            T.lv[wy + 1][lx] = ltp(L2, pk);
             if (h_kind >= 0) {
                 const long long q = pk + h_off;
                 switch (h_kind) {
@@ -463,56 +478,65 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
                     cyp += T.gy[0][q][wy + 2 - q][lx];
                     cyv += T.gy[1][q][wy + 2 - q][lx];
                 }
-                xa[2][0] = cxp + cyp;
-                xa[2][1] = cxu;
-                xa[2][2] = cyv;
+                xa[Z2][0] = cxp + cyp;
+                xa[Z2][1] = cxu;
+                xa[Z2][2] = cyv;
             }
         }
         // ---- z pair of cell k (output rows)
         if (out_row && k >= 2 && k <= Nz - 3) {
             R mpL, mpR, mwL, mwR, t0, t1, t2, t3;
-            godunov_vjp(ap[2] - ap[1], aw[2] - aw[1], K.kp, K.kc, K.kr, mpL, t0, mwL, t1);
-            godunov_vjp(ap[1] - ap[0], aw[1] - aw[0], K.kp, K.kc, K.kr, t2, mpR, t3, mwR);
+            godunov_vjp(ap[Z2] - ap[Z1], aw[Z2] - aw[Z1], K.kp, K.kc, K.kr, mpL, t0, mwL, t1);
+            godunov_vjp(ap[Z1] - ap[Z0], aw[Z1] - aw[Z0], K.kp, K.kc, K.kr, t2, mpR, t3, mwR);
             R gp[5], gw[5];
-            weno_pair_vjp(zp[0], zp[1], zp[2], zp[3], zp[4], mpL, mpR, gp);
-            weno_pair_vjp(zw[0], zw[1], zw[2], zw[3], zw[4], mwL, mwR, gw);
-#pragma unroll
-            for (int q = 0; q < 5; ++q) { za[q] += gp[q]; zb[q] += gw[q]; }
+            weno_pair_vjp(zp[Z0], zp[Z1], zp[Z2], zp[Z3], zp[Z4], mpL, mpR, gp);
+            weno_pair_vjp(zw[Z0], zw[Z1], zw[Z2], zw[Z3], zw[Z4], mwL, mwR, gw);
+            za[Z0] += gp[0]; za[Z1] += gp[1]; za[Z2] += gp[2]; za[Z3] += gp[3]; za[Z4] += gp[4];
+            zb[Z0] += gw[0]; zb[Z1] += gw[1]; zb[Z2] += gw[2]; zb[Z3] += gw[3]; zb[Z4] += gw[4];
         }
         // ---- plane k - 2 is complete
         const int m = k - 2;
         if (out_cell && m >= k0 && m < kend) {
-            const long long C = nxy * (long long)m + col;
-            const bool me = upd[C] != 0;
-            const R o0 = L0[C], o1 = L1[C], o2 = L2[C], o3 = L3[C];
-            const R v0 = xa[0][0] + za[0] + (me ? K.beta * o0 : R(0));
-            const R v1 = xa[0][1] + (me ? K.beta * o1 : R(0));
-            const R v2 = xa[0][2] + (me ? K.beta * o2 : R(0));
-            const R v3 = zb[0] + (me ? K.beta * o3 : R(0));
-            lin0[C] = ACC_IN ? lin0[C] + v0 : v0;
-            lin1[C] = ACC_IN ? lin1[C] + v1 : v1;
-            lin2[C] = ACC_IN ? lin2[C] + v2 : v2;
-            lin3[C] = ACC_IN ? lin3[C] + v3 : v3;
+            const long long pm = nxy * (long long)m;
+            const bool me = uld(pm) != 0;
+            const R o0 = rld(L0, pm), o1 = rld(L1, pm), o2 = rld(L2, pm), o3 = rld(L3, pm);
+            const R v0 = xa[Z0][0] + za[Z0] + (me ? K.beta * o0 : R(0));
+            const R v1 = xa[Z0][1] + (me ? K.beta * o1 : R(0));
+            const R v2 = xa[Z0][2] + (me ? K.beta * o2 : R(0));
+            const R v3 = zb[Z0] + (me ? K.beta * o3 : R(0));
+            rst(lin0, pm, ACC_IN ? rld(lin0, pm) + v0 : v0);
+            rst(lin1, pm, ACC_IN ? rld(lin1, pm) + v1 : v1);
+            rst(lin2, pm, ACC_IN ? rld(lin2, pm) + v2 : v2);
+            rst(lin3, pm, ACC_IN ? rld(lin3, pm) + v3 : v3);
             if (HAS_BASE) {
                 const R w0 = me ? K.alpha * o0 : R(0), w1 = me ? K.alpha * o1 : R(0);
                 const R w2 = me ? K.alpha * o2 : R(0), w3 = me ? K.alpha * o3 : R(0);
-                lb0[C] = ACC_BASE ? lb0[C] + w0 : w0;
-                lb1[C] = ACC_BASE ? lb1[C] + w1 : w1;
-                lb2[C] = ACC_BASE ? lb2[C] + w2 : w2;
-                lb3[C] = ACC_BASE ? lb3[C] + w3 : w3;
+                rst(lb0, pm, ACC_BASE ? rld(lb0, pm) + w0 : w0);
+                rst(lb1, pm, ACC_BASE ? rld(lb1, pm) + w1 : w1);
+                rst(lb2, pm, ACC_BASE ? rld(lb2, pm) + w2 : w2);
+                rst(lb3, pm, ACC_BASE ? rld(lb3, pm) + w3 : w3);
             }
         }
-        // ---- advance the rings
-#pragma unroll
-        for (int q = 0; q < 4; ++q) { za[q] = za[q + 1]; zb[q] = zb[q + 1]; zp[q] = zp[q + 1]; zw[q] = zw[q + 1]; }
-        za[4] = 0; zb[4] = 0;
-        zp[4] = np_; zw[4] = nw_;
-        ap[0] = ap[1]; ap[1] = ap[2]; ap[2] = nap;
-        aw[0] = aw[1]; aw[1] = aw[2]; aw[2] = naw;
-#pragma unroll
-        for (int f = 0; f < 3; ++f) { xa[0][f] = xa[1][f]; xa[1][f] = xa[2][f]; xa[2][f] = 0; }
+        // ---- advance the rings: logical slot 0 leaves, the new entries take free slots
+        za[Z0] = 0; zb[Z0] = 0;                       // becomes plane k+3's (logical 4)
+        zp[Z0] = np_; zw[Z0] = nw_;                   // plane k+3
+        ap[Z3] = nap; aw[Z3] = naw;                   // plane k+2 (logical 2 next phase)
+        xa[Z3][0] = 0; xa[Z3][1] = 0; xa[Z3][2] = 0;  // plane k+1 (logical 2 next phase)
+    };
+    // five planes per trip (the ring period); the plane range is block-uniform
+    for (int k = k0 - 2; k <= kend + 1; k += 5) {
+        step(k, std::integral_constant<int, 0>{});
+        if (k + 1 > kend + 1) break;
+        step(k + 1, std::integral_constant<int, 1>{});
+        if (k + 2 > kend + 1) break;
+        step(k + 2, std::integral_constant<int, 2>{});
+        if (k + 3 > kend + 1) break;
+        step(k + 3, std::integral_constant<int, 3>{});
+        if (k + 4 > kend + 1) break;
+        step(k + 4, std::integral_constant<int, 4>{});
     }
 }
+
 
 // ADE replay of one step on the boundary list: x1 = stage 0 (x0, wout(V0)),
 // x2 = stage 1 (x0, x1, wout(V1)), the same arithmetic as the forward boundary kernel.
