@@ -249,12 +249,22 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
 #pragma unroll
         for (int q = 0; q < 5; ++q) { pz[q] = p7[q + 2]; wz[q] = w7[q + 2]; }
     }
-    // plane values in flight: own u, v, halo items
+    // plane values in flight: own u, v, update flag, base set (stages 1, 2), halo items;
+    // all issued one plane ahead so their HBM latency hides under the previous plane
     R nu, nv, nh[NH];
+    R nb0 = 0, nb1 = 0, nb2 = 0, nb3 = 0;
+    unsigned nflag;
     {
         const long long pl = nxy * (long long)k0;
         nu = LD(iu, pl, sc.s[1]);
         nv = LD(iv, pl, sc.s[2]);
+        nflag = plane_ld_u8(upd + pl, (unsigned)nxy, col);
+        if (STAGE != 0) {
+            nb0 = LD(bp, pl, sc.s[0]);
+            nb1 = LD(bu, pl, sc.s[1]);
+            nb2 = LD(bv, pl, sc.s[2]);
+            nb3 = LD(bw, pl, sc.s[3]);
+        }
 #pragma unroll
         for (int q = 0; q < NH; ++q)                       // per-lane field: no uniform base
             nh[q] = h_on[q] ? CV::ld((hsrc[q] + pl)[h_off[q]], hsc[q]) : R(0);
@@ -267,6 +277,8 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         const long long pl = nxy * (long long)k;
         // ---- fill the plane tiles from the values loaded during the previous plane
         const R pc = pz[(1 + PH) % 5], uc = nu, vc = nv, wc = wz[(1 + PH) % 5];
+        const R p0 = nb0, u0 = nb1, v0 = nb2, w0 = nb3;
+        const unsigned flag = nflag;
         L.p[wy + 3][lx + 2] = pc;
         L.u[wy][lx + 2] = uc;
         L.v[wy + 3][lx] = vc;
@@ -280,6 +292,13 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
             const long long pn = pl + nxy;
             nu = LD(iu, pn, sc.s[1]);
             nv = LD(iv, pn, sc.s[2]);
+            nflag = plane_ld_u8(upd + pn, (unsigned)nxy, col);
+            if (STAGE != 0) {
+                nb0 = LD(bp, pn, sc.s[0]);
+                nb1 = LD(bu, pn, sc.s[1]);
+                nb2 = LD(bv, pn, sc.s[2]);
+                nb3 = LD(bw, pn, sc.s[3]);
+            }
 #pragma unroll
             for (int q = 0; q < NH; ++q)
                 if (h_on[q]) nh[q] = CV::ld((hsrc[q] + pn)[h_off[q]], hsc[q]);
@@ -335,7 +354,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         weno_pair(wz[(0 + PH) % 5], wz[(1 + PH) % 5], wz[(2 + PH) % 5], wz[(3 + PH) % 5], wz[(4 + PH) % 5], wAn,
                   wBn);
         godunov_f(pA, pBn, wA, wBn, K, FzP_p, FzP_w);
-        if (out_lane && plane_ld_u8(upd + pl, (unsigned)nxy, col) == 1) {
+        if (out_lane && flag == 1) {
             const R sp = -FxP_p + FxM_p - FyP_p + FyM_p - FzP_p + FzP;
             const R Du = -FxP_u + FxM_u, Dv = -FyP_v + FyM_v, Dw = -FzP_w + FzW;
             R uo, vo, wo, po, wgt;
@@ -346,8 +365,6 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
                 po = fma_s(K.dth, sp, pc);
                 wgt = R(1);
             } else {
-                const R p0 = LD(bp, pl, sc.s[0]), u0 = LD(bu, pl, sc.s[1]);
-                const R v0 = LD(bv, pl, sc.s[2]), w0 = LD(bw, pl, sc.s[3]);
                 if (STAGE == 1) {
                     uo = fma(R(0.75), u0, R(0.25) * fma_s(K.dth, Du, uc));
                     vo = fma(R(0.75), v0, R(0.25) * fma_s(K.dth, Dv, vc));
