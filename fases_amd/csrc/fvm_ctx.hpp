@@ -173,19 +173,20 @@ int launch_tiled(fases_ctx *ctx, int stage, const FieldSet<S> &fi, const FieldSe
     (void)rho; (void)c; (void)h;
     const TileK<R> K = tile_constants<R>(ctx->prm);
     const dim3 tb(64, NW);
-    const dim3 tg((ctx->Nx - 6 + tile::TXO - 1) / tile::TXO, (ctx->Ny - 6 + NW - 1) / NW,
-                  (ctx->Nz - 6 + tz - 1) / tz);
+    const int nbx = (ctx->Nx - 6 + tile::TXO - 1) / tile::TXO, nby = (ctx->Ny - 6 + NW - 1) / NW;
+    const int nbz = (ctx->Nz - 6 + tz - 1) / tz;
+    const dim3 tg((unsigned)nbx * (unsigned)nby * (unsigned)nbz);     // 1-D: tiles ordered per XCD in the kernel
     const Scl<R> sc = field_scales<R>(ctx);
     const TileFields<S> F{fi.p, fi.u, fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w};
     if (stage == 0)
         hipLaunchKernelGGL((k_flux_tiled<S, R, 0, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
-                           ctx->d_upd, ctx->pml, K, dt, sc, tz);
+                           ctx->d_upd, ctx->pml, K, dt, sc, tz, nbx, nby);
     else if (stage == 1)
         hipLaunchKernelGGL((k_flux_tiled<S, R, 1, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
-                           ctx->d_upd, ctx->pml, K, dt, sc, tz);
+                           ctx->d_upd, ctx->pml, K, dt, sc, tz, nbx, nby);
     else
         hipLaunchKernelGGL((k_flux_tiled<S, R, 2, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
-                           ctx->d_upd, ctx->pml, K, dt, sc, tz);
+                           ctx->d_upd, ctx->pml, K, dt, sc, tz, nbx, nby);
     return FASES_OK;
 }
 

@@ -409,12 +409,26 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     }
 }
 
+// XCD-aware tile order. Workgroups of a launch go to the 8 XCDs round-robin (id mod 8),
+// and each XCD has its own L2: tile t of the x-fastest tile order is given to workgroup
+// w such that XCD x works through the contiguous tile range [x q + min(x, r), ...) of
+// q + (x < r) tiles (q = n / 8, r = n mod 8). Neighbouring tiles then share an L2 and
+// its copy of their common halo rows and planes.
+constexpr unsigned kXcds = 8;
+__device__ inline unsigned xcd_tile(unsigned w, unsigned n) {
+    const unsigned q = n / kXcds, r = n % kXcds, x = w % kXcds, loc = w / kXcds;
+    return x * q + min(x, r) + loc;
+}
+
 template <typename S, typename R, int STAGE, int NW>
 __global__ void __launch_bounds__(64 * NW)
 k_flux_tiled(int Nx, int Ny, int Nz, TileFields<S> F, const uint8_t *__restrict__ upd, PmlSet pml, TileK<R> K, R dt,
-             Scl<R> sc, int tz) {
+             Scl<R> sc, int tz, int nbx, int nby) {
     __shared__ TileLds<R, NW> L;
-    flux_tile<S, R, STAGE, NW>(L, Nx, Ny, Nz, F, upd, pml, K, dt, sc, tz, blockIdx.x, blockIdx.y, blockIdx.z);
+    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = (int)(t % (unsigned)nbx), by = (int)((t / (unsigned)nbx) % (unsigned)nby);
+    const int bz = (int)(t / ((unsigned)nbx * (unsigned)nby));
+    flux_tile<S, R, STAGE, NW>(L, Nx, Ny, Nz, F, upd, pml, K, dt, sc, tz, bx, by, bz);
 }
 
 }  // namespace fasesdev
