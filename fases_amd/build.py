@@ -29,7 +29,7 @@ ARCH = os.environ.get("FASES_ARCH", "gfx950")
 HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 HIP_FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-munsafe-fp-atomics", "-Wno-unused-result"]
+             "-munsafe-fp-atomics", "-Wno-unused-result", "-Wno-cuda-compat"]
 
 HOST_SOURCES = ["host_formats.cpp", "host_geometry.cpp"]
 HIP_SOURCES = ["fvm_context.hip", "fvm_batch.hip", "fvm_adjoint.hip"]
