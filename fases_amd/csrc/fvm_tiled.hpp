@@ -185,6 +185,9 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     auto LD = [&](const S *f, long long pl, R scale) __device__ {
         return CV::ld(PlaneIO<S>::ld(plane_rsrc(f + pl, pbytes), cb), scale);
     };
+    // ... the stored value itself (converted where it is used, so the wait on the load
+    // lands there and not right after issue)
+    auto LDR = [&](const S *f, long long pl) __device__ { return PlaneIO<S>::ld(plane_rsrc(f + pl, pbytes), cb); };
     auto ST = [&](S *f, long long pl, R x, R inv_scale) __device__ {
         PlaneIO<S>::st(plane_rsrc(f + pl, pbytes), cb, CV::st(x, inv_scale));
     };
@@ -233,9 +236,9 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     R pz[5], wz[5];
     R pA, wA, FzP, FzW;
     {
-        R p7[7], w7[7];
+        R p7[6], w7[6];
 #pragma unroll
-        for (int q = 0; q < 7; ++q) {
+        for (int q = 0; q < 6; ++q) {
             const long long pq = nxy * (long long)(k0 - 3 + q);
             p7[q] = LD(ip, pq, sc.s[0]);
             w7[q] = LD(iw, pq, sc.s[3]);
@@ -247,27 +250,38 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         weno_pair(w7[1], w7[2], w7[3], w7[4], w7[5], wA, wB0);
         godunov_f(pAm, pB0, wAm, wB0, K, FzP, FzW);            // face k0-1/2
 #pragma unroll
-        for (int q = 0; q < 5; ++q) { pz[q] = p7[q + 2]; wz[q] = w7[q + 2]; }
+        for (int q = 0; q < 4; ++q) { pz[q] = p7[q + 2]; wz[q] = w7[q + 2]; }
+        pz[4] = wz[4] = R(0);                  // plane k0+3: from pend_p / pend_w below
     }
-    // plane values in flight: own u, v, update flag, base set (stages 1, 2), halo items;
-    // all issued one plane ahead so their HBM latency hides under the previous plane
-    R nu, nv, nh[NH];
-    R nb0 = 0, nb1 = 0, nb2 = 0, nb3 = 0;
+    // values in flight, in storage precision: own u, v, update flag, base set (stages 1,
+    // 2), halo items, and p, w of plane k+3 for the z window. Each plane converts what
+    // the previous plane loaded and at once issues the loads of the next one, so a whole
+    // plane of work covers their HBM latency.
+    constexpr bool kRaw = !std::is_same<S, R>::value;      // storage needs a conversion
+    S nu_s, nv_s, nh_s[NH], pend_p, pend_w;
+    S nb0 = S(), nb1 = S(), nb2 = S(), nb3 = S();
     unsigned nflag;
     {
         const long long pl = nxy * (long long)k0;
-        nu = LD(iu, pl, sc.s[1]);
-        nv = LD(iv, pl, sc.s[2]);
+        nu_s = LDR(iu, pl);
+        nv_s = LDR(iv, pl);
         nflag = plane_ld_u8(upd + pl, (unsigned)nxy, col);
         if (STAGE != 0) {
-            nb0 = LD(bp, pl, sc.s[0]);
-            nb1 = LD(bu, pl, sc.s[1]);
-            nb2 = LD(bv, pl, sc.s[2]);
-            nb3 = LD(bw, pl, sc.s[3]);
+            nb0 = LDR(bp, pl);
+            nb1 = LDR(bu, pl);
+            nb2 = LDR(bv, pl);
+            nb3 = LDR(bw, pl);
         }
 #pragma unroll
         for (int q = 0; q < NH; ++q)                       // per-lane field: no uniform base
-            nh[q] = h_on[q] ? CV::ld((hsrc[q] + pl)[h_off[q]], hsc[q]) : R(0);
+            nh_s[q] = h_on[q] ? (hsrc[q] + pl)[h_off[q]] : S();
+        const long long p3 = nxy * (long long)(k0 + 3);
+        pend_p = LDR(ip, p3);
+        pend_w = LDR(iw, p3);
+        if constexpr (!kRaw) {                 // this order fills slots at the plane's end
+            pz[4] = CV::ld(pend_p, sc.s[0]);
+            wz[4] = CV::ld(pend_w, sc.s[3]);
+        }
     }
 
     // One plane k. PH = (k - k0) mod 5 is a compile-time constant: logical window slot q
@@ -275,38 +289,53 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     auto plane = [&](int k, auto phase) __device__ __attribute__((always_inline)) {
         constexpr int PH = decltype(phase)::value;
         const long long pl = nxy * (long long)k;
-        // ---- fill the plane tiles from the values loaded during the previous plane
-        const R pc = pz[(1 + PH) % 5], uc = nu, vc = nv, wc = wz[(1 + PH) % 5];
-        const R p0 = nb0, u0 = nb1, v0 = nb2, w0 = nb3;
+        // ---- the values the previous plane loaded, into compute precision. With reduced
+        // storage the z window takes plane k+3 into its logical slot 4 (the slot plane
+        // k-2 held) here, and the next plane's loads go out at once, a whole plane ahead
+        // of their conversion; with FP32 / FP64 storage (no conversion) the loads go out
+        // after the first barrier and the slot is filled at the end of the plane
+        // (measured faster each way).
+        if constexpr (kRaw) {
+            pz[(PH + 4) % 5] = CV::ld(pend_p, sc.s[0]);
+            wz[(PH + 4) % 5] = CV::ld(pend_w, sc.s[3]);
+        }
+        const R pc = pz[(1 + PH) % 5], wc = wz[(1 + PH) % 5];
+        const R uc = CV::ld(nu_s, sc.s[1]), vc = CV::ld(nv_s, sc.s[2]);
+        const S b0 = nb0, b1 = nb1, b2 = nb2, b3 = nb3;
         const unsigned flag = nflag;
+        R hv[NH];
+#pragma unroll
+        for (int q = 0; q < NH; ++q) hv[q] = h_on[q] ? CV::ld(nh_s[q], hsc[q]) : R(0);
+        // ---- the next plane's loads (p, w of plane k+4 for the z window)
+        auto issue_next = [&]() __device__ __attribute__((always_inline)) {
+            if (k + 1 >= kend) return;
+            const long long pn = pl + nxy;
+            nu_s = LDR(iu, pn);
+            nv_s = LDR(iv, pn);
+            nflag = plane_ld_u8(upd + pn, (unsigned)nxy, col);
+            if (STAGE != 0) {
+                nb0 = LDR(bp, pn);
+                nb1 = LDR(bu, pn);
+                nb2 = LDR(bv, pn);
+                nb3 = LDR(bw, pn);
+            }
+#pragma unroll
+            for (int q = 0; q < NH; ++q)
+                if (h_on[q]) nh_s[q] = (hsrc[q] + pn)[h_off[q]];
+            const long long p4 = nxy * (long long)(k + 4);
+            pend_p = LDR(ip, p4);
+            pend_w = LDR(iw, p4);
+        };
+        if constexpr (kRaw) issue_next();
+        // ---- fill the plane tiles
         L.p[wy + 3][lx + 2] = pc;
         L.u[wy][lx + 2] = uc;
         L.v[wy + 3][lx] = vc;
 #pragma unroll
         for (int q = 0; q < NH; ++q)
-            if (h_on[q]) *hdst[q] = nh[q];
+            if (h_on[q]) *hdst[q] = hv[q];
         __syncthreads();
-        // ---- prefetch the next plane
-        R pnew = 0, wnew = 0;
-        if (k + 1 < kend) {
-            const long long pn = pl + nxy;
-            nu = LD(iu, pn, sc.s[1]);
-            nv = LD(iv, pn, sc.s[2]);
-            nflag = plane_ld_u8(upd + pn, (unsigned)nxy, col);
-            if (STAGE != 0) {
-                nb0 = LD(bp, pn, sc.s[0]);
-                nb1 = LD(bu, pn, sc.s[1]);
-                nb2 = LD(bv, pn, sc.s[2]);
-                nb3 = LD(bw, pn, sc.s[3]);
-            }
-#pragma unroll
-            for (int q = 0; q < NH; ++q)
-                if (h_on[q]) nh[q] = CV::ld((hsrc[q] + pn)[h_off[q]], hsc[q]);
-            // z window: plane k+4
-            const long long p4 = nxy * (long long)(k + 4);
-            pnew = LD(ip, p4, sc.s[0]);
-            wnew = LD(iw, p4, sc.s[3]);
-        }
+        if constexpr (!kRaw) issue_next();
         // ---- y states of the own row, and of the halo rows on four waves
         {
             R A, B, Av, Bv;
@@ -365,6 +394,8 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
                 po = fma_s(K.dth, sp, pc);
                 wgt = R(1);
             } else {
+                const R p0 = CV::ld(b0, sc.s[0]), u0 = CV::ld(b1, sc.s[1]);
+                const R v0 = CV::ld(b2, sc.s[2]), w0 = CV::ld(b3, sc.s[3]);
                 if (STAGE == 1) {
                     uo = fma(R(0.75), u0, R(0.25) * fma_s(K.dth, Du, uc));
                     vo = fma(R(0.75), v0, R(0.25) * fma_s(K.dth, Dv, vc));
@@ -389,10 +420,12 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
             ST(op, pl, po, sc.is[0]);
         }
         // advance the z pipeline: the window is a ring indexed at compile time, so the
-        // advance moves no registers (slot 0, plane k-1, becomes slot 4, plane k+4)
+        // advance moves no registers (plane k+4 takes slot PH, which plane k-1 held)
         pA = pAn; wA = wAn; FzP = FzP_p; FzW = FzP_w;
-        pz[PH] = pnew;
-        wz[PH] = wnew;
+        if constexpr (!kRaw) {
+            pz[PH] = CV::ld(pend_p, sc.s[0]);
+            wz[PH] = CV::ld(pend_w, sc.s[3]);
+        }
     };
     // five planes per trip so the ring returns to its start; kend is uniform, so every
     // wave leaves at the same plane and the barriers stay matched
