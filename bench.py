@@ -38,6 +38,43 @@ def log(msg: str) -> None:
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def geometry_timing(room, h) -> dict:
+    """Scene classification (signed distance + material id) on the host (OpenMP) and on
+    the GPU, with the results compared bit for bit: the bench room, and a room of the
+    same cell count per side ratio meshed with 12 n^2 triangles."""
+    import tempfile
+
+    from fases_amd.solver import Scene
+    from fases_amd.workloads import write_tessellated_room
+
+    def both(make):
+        t = time.time()
+        a = make(None)
+        th = time.time() - t
+        t = time.time()
+        b = make(0)
+        tg = time.time() - t
+        same = bool(np.array_equal(a.S.view(np.uint32), b.S.view(np.uint32)) and np.array_equal(a.oid, b.oid))
+        return {"host_s": round(th, 3), "gpu_s": round(tg, 3), "speedup": round(th / tg, 1), "bitwise_equal": same,
+                "cells": int(np.prod(a.dims))}
+
+    Scene.shoebox(8, 8, 8, h, device=0)         # device initialised outside the timing
+    out = {"room": "shoebox %s cells dx=%g (12 triangles)" % ("x".join(map(str, room)), h),
+           **both(lambda d: Scene.shoebox(*room, h, device=d))}
+    tess = (256, 192, 128)
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "tess.obj")
+        nt = write_tessellated_room(p, tess[0] * 0.02, tess[1] * 0.02, tess[2] * 0.02, 11)
+        out["tessellated"] = {"room": "%s cells dx=0.02, walls split into 11x11 quads (%d triangles)"
+                                      % ("x".join(map(str, tess)), nt),
+                              **both(lambda d: Scene.from_obj(p, 0.02, device=d))}
+    out["host_threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log("geometry: bench room host %.2f s / GPU %.3f s (equal %s); tessellated host %.2f s / GPU %.3f s (equal %s)"
+        % (out["host_s"], out["gpu_s"], out["bitwise_equal"], out["tessellated"]["host_s"],
+           out["tessellated"]["gpu_s"], out["tessellated"]["bitwise_equal"]))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +89,7 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=3000, help="steps of the adjoint benchmark (0: skip)")
     ap.add_argument("--tile-z", type=int, default=0, help="flux kernel z depth per block (0: autotune)")
     ap.add_argument("--tile-rows", type=int, default=0, help="flux kernel waves per block, 8 or 16 (0: autotune)")
+    ap.add_argument("--no-geometry", action="store_true", help="skip the host vs GPU scene classification timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the FVM core has no multi-GPU path in this build "
@@ -187,6 +225,10 @@ def main() -> int:
         log("adjoint: forward %.0f ms, gradient %.0f ms (x%.2f)" % (fwd_ms, (t_c - t_a) * 1e3,
                                                                   adjoint["gradient_over_forward"]))
 
+    geometry = None
+    if not args.no_geometry:
+        geometry = geometry_timing(room, wl.h)
+
     cpu = None
     if args.cpu_steps > 0:
         from oracle import oracle as O
@@ -227,6 +269,7 @@ def main() -> int:
         "fp16_storage": fp16,
         "batched": batch,
         "adjoint": adjoint,
+        "geometry": geometry,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -77,6 +77,8 @@ SIGNATURES = {
     "fases_write_f32": (C.c_int, [C.c_char_p, _dbl, C.c_int64]),
     "fases_scene_load_obj": (C.c_int, [C.c_char_p, C.c_float, _P(C.c_void_p)]),
     "fases_scene_shoebox": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float, _P(C.c_void_p)]),
+    "fases_scene_load_obj_dev": (C.c_int, [C.c_char_p, C.c_float, C.c_int, _P(C.c_void_p)]),
+    "fases_scene_shoebox_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, _P(C.c_void_p)]),
     "fases_scene_info": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int), _P(C.c_int), _flt]),
     "fases_scene_group_name": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int]),
     "fases_scene_get": (C.c_int, [C.c_void_p, _flt, _u8]),
@@ -94,6 +96,7 @@ SIGNATURES = {
     "fases_reset": (C.c_int, [C.c_void_p]),
     "fases_run": (C.c_int, [C.c_void_p, C.c_int64, _dbl, _dbl, _dbl, _dbl]),
     "fases_info": (C.c_int, [C.c_void_p, _i64, _i64, _i64, _i64]),
+    "fases_device_count": (C.c_int, [_P(C.c_int)]),
     "fases_tile_config": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
     "fases_batch_run": (C.c_int, [_P(C.c_void_p), C.c_int, _i64, _P(_P(C.c_double)), _P(C.c_double)]),
     "fases_adjoint_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _P(C.c_double)]),

@@ -32,7 +32,9 @@ HIP_FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
              "-munsafe-fp-atomics", "-Wno-unused-result", "-Wno-cuda-compat"]
 
 HOST_SOURCES = ["host_formats.cpp", "host_geometry.cpp"]
-HIP_SOURCES = ["fvm_context.hip", "fvm_batch.hip", "fvm_adjoint.hip"]
+HIP_SOURCES = ["fvm_context.hip", "fvm_batch.hip", "fvm_adjoint.hip", "fvm_geometry.hip"]
+# The geometry kernels reproduce the host classification bit for bit: no FMA contraction.
+HIP_EXTRA = {"fvm_geometry.hip": ["-ffp-contract=off"]}
 
 
 def _run(cmd: list[str]) -> None:
@@ -49,7 +51,7 @@ def build_core() -> Path:
         objs.append(str(o))
     for src in HIP_SOURCES:
         o = OBJ / (Path(src).stem + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
+        _run([HIPCC, *HIP_FLAGS, *HIP_EXTRA.get(src, []), "-c", str(CSRC / src), "-o", str(o)])
         objs.append(str(o))
     out = PKG / "libfases_core.so"
     tmp = out.with_suffix(".so.tmp")

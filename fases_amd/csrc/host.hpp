@@ -41,6 +41,16 @@ struct Scene {
 
 bool mesh_from_obj(const std::string &path, Mesh &m, std::string &err);
 void mesh_shoebox(int nx, int ny, int nz, float h, Mesh &m);
-bool scene_build(Scene &sc, std::string &err);   // placement, grid dims, SDF, OID
+// Placement, grid dims, SDF, OID. device < 0: on the host; otherwise steps SDF and OID
+// run on that HIP device (same results bit for bit). On failure *code (if given) is
+// the FASES_E_* code.
+bool scene_build(Scene &sc, std::string &err, int device = -1, int *code = nullptr);
+
+// Device part of scene_build (fvm_geometry.hip): triangle corners, band boxes
+// (i0, i1, j0, j1, k0, k1 per triangle), group per triangle and the group -> material
+// id table in; S and oid (Nx Ny Nz each) out.
+int classify_gpu(int device, int Nx, int Ny, int Nz, float h, const std::vector<F3> &ta, const std::vector<F3> &tb,
+                 const std::vector<F3> &tc, const std::vector<int> &boxes, const std::vector<int> &gid,
+                 const std::vector<int> &lid_of, int max_gid, float *S_out, uint8_t *oid_out, std::string &err);
 
 }  // namespace fases

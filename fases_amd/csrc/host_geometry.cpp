@@ -199,9 +199,10 @@ void mesh_shoebox(int nx, int ny, int nz, float h, Mesh &m) {
     m.hi = mk(X, Y, Z);
 }
 
-bool scene_build(Scene &sc, std::string &err) {
+bool scene_build(Scene &sc, std::string &err, int device, int *code) {
     Mesh &m = sc.mesh;
     const float h = sc.h;
+    if (code) *code = FASES_E_ARG;
     if (!(h > 0.f)) { err = "grid spacing h must be > 0"; return false; }
     const float pad = 3 * h;   // PARSE3D_GHOST cells of ghost padding per side
     m.offset = mk(-m.lo.x + pad, -m.lo.y + pad, -m.lo.z + pad);
@@ -231,6 +232,21 @@ bool scene_build(Scene &sc, std::string &err) {
         ta[t] = placed(m, m.tris[t].a); tb[t] = placed(m, m.tris[t].b); tc[t] = placed(m, m.tris[t].c);
     }
 
+    // Material id of each mesh group: 1-based over the groups that have faces, in group
+    // order; at most 254 ids (parsing3D.h:966-977).
+    int max_gid = -1;
+    for (const Triangle &t : m.tris) max_gid = std::max(max_gid, t.group);
+    std::vector<int> lid_of((size_t)std::max(max_gid + 1, 0), -1);
+    {
+        int next = 0;
+        for (int g = 0; g <= max_gid; ++g) {
+            bool used = false;
+            for (const Triangle &t : m.tris) if (t.group == g) { used = true; break; }
+            if (!used) continue;
+            if (next <= 254) lid_of[(size_t)g] = 1 + next++;
+        }
+    }
+
     // 1) |distance| clamped at the band radius 8h, min over triangles whose expanded
     //    bounding box covers the cell centre. Min is order-free, so z-slabs run in parallel.
     const float band = 8.f * h;
@@ -246,6 +262,20 @@ bool scene_build(Scene &sc, std::string &err) {
         int k0 = (int)std::floor((loz - 0.f) / h - 0.5f), k1 = (int)std::ceil((hiz - 0.f) / h - 0.5f);
         bi0[t] = std::max(i0, 0); bj0[t] = std::max(j0, 0); bk0[t] = std::max(k0, 0);
         bi1[t] = std::min(i1, Nx - 1); bj1[t] = std::min(j1, Ny - 1); bk1[t] = std::min(k1, Nz - 1);
+    }
+    if (device >= 0) {                     // steps 1-3 on the GPU
+        std::vector<int> boxes(6 * nt), gid(nt);
+        for (size_t t = 0; t < nt; ++t) {
+            const int b6[6] = {bi0[t], bi1[t], bj0[t], bj1[t], bk0[t], bk1[t]};
+            std::copy(b6, b6 + 6, boxes.begin() + 6 * t);
+            gid[t] = m.tris[t].group;
+        }
+        sc.S.assign(N, 0.f);
+        sc.oid.assign(N, (uint8_t)0xFF);
+        const int rc = classify_gpu(device, Nx, Ny, Nz, h, ta, tb, tc, boxes, gid, lid_of, max_gid, sc.S.data(),
+                                    sc.oid.data(), err);
+        if (rc) { if (code) *code = rc; return false; }
+        return true;
     }
 #pragma omp parallel for schedule(dynamic, 1)
     for (int k = 0; k < Nz; ++k) {
@@ -301,17 +331,7 @@ bool scene_build(Scene &sc, std::string &err) {
     //    (strictly smaller distance wins, so the first triangle keeps ties).
     const float tol = 3.0f * h;
     sc.oid.assign(N, (uint8_t)0xFF);
-    int max_gid = -1;
-    for (const Triangle &t : m.tris) max_gid = std::max(max_gid, t.group);
     if (max_gid < 0) return true;
-    std::vector<int> lid_of((size_t)max_gid + 1, -1);
-    int next = 0;
-    for (int g = 0; g <= max_gid; ++g) {
-        bool used = false;
-        for (const Triangle &t : m.tris) if (t.group == g) { used = true; break; }
-        if (!used) continue;
-        if (next <= 254) lid_of[(size_t)g] = 1 + next++;
-    }
     const float two_h = 2.0f * h;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int k = 1; k < Nz - 1; ++k) {
@@ -351,6 +371,14 @@ using namespace fases;
 extern "C" {
 
 int fases_scene_load_obj(const char *path, float h, fases_scene **out) {
+    return fases_scene_load_obj_dev(path, h, -1, out);
+}
+
+int fases_scene_shoebox(int nx, int ny, int nz, float h, fases_scene **out) {
+    return fases_scene_shoebox_dev(nx, ny, nz, h, -1, out);
+}
+
+int fases_scene_load_obj_dev(const char *path, float h, int device, fases_scene **out) {
     if (!path || !out) { set_global_error("fases_scene_load_obj: null argument"); return FASES_E_ARG; }
     *out = nullptr;
     Scene *sc = new (std::nothrow) Scene();
@@ -362,7 +390,8 @@ int fases_scene_load_obj(const char *path, float h, fases_scene **out) {
     }
     sc->h = h;
     try {
-        if (!scene_build(*sc, err)) { set_global_error(err); delete sc; return FASES_E_ARG; }
+        int code = FASES_E_ARG;
+        if (!scene_build(*sc, err, device, &code)) { set_global_error(err); delete sc; return code; }
     } catch (const std::bad_alloc &) {
         set_global_error("out of host memory building the scene"); delete sc; return FASES_E_NOMEM;
     }
@@ -370,7 +399,7 @@ int fases_scene_load_obj(const char *path, float h, fases_scene **out) {
     return FASES_OK;
 }
 
-int fases_scene_shoebox(int nx, int ny, int nz, float h, fases_scene **out) {
+int fases_scene_shoebox_dev(int nx, int ny, int nz, float h, int device, fases_scene **out) {
     if (!out || nx < 1 || ny < 1 || nz < 1) { set_global_error("fases_scene_shoebox: bad argument"); return FASES_E_ARG; }
     *out = nullptr;
     Scene *sc = new (std::nothrow) Scene();
@@ -379,7 +408,8 @@ int fases_scene_shoebox(int nx, int ny, int nz, float h, fases_scene **out) {
     sc->h = h;
     std::string err;
     try {
-        if (!scene_build(*sc, err)) { set_global_error(err); delete sc; return FASES_E_ARG; }
+        int code = FASES_E_ARG;
+        if (!scene_build(*sc, err, device, &code)) { set_global_error(err); delete sc; return code; }
     } catch (const std::bad_alloc &) {
         set_global_error("out of host memory building the scene"); delete sc; return FASES_E_NOMEM;
     }

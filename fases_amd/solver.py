@@ -200,15 +200,18 @@ class Scene:
         check(lib.fases_scene_get(handle, _ptr(self.S, C.c_float), _ptr(self.oid, C.c_uint8)))
 
     @classmethod
-    def from_obj(cls, path: str, h: float) -> "Scene":
+    def from_obj(cls, path: str, h: float, device: int | None = None) -> "Scene":
+        """device None: classified on the host; an index: on that GPU (same S and oid)."""
         hd = C.c_void_p()
-        check(load().fases_scene_load_obj(path.encode(), h, C.byref(hd)))
+        dev = -1 if device is None else int(device)
+        check(load().fases_scene_load_obj_dev(path.encode(), h, dev, C.byref(hd)))
         return cls(hd)
 
     @classmethod
-    def shoebox(cls, nx: int, ny: int, nz: int, h: float) -> "Scene":
+    def shoebox(cls, nx: int, ny: int, nz: int, h: float, device: int | None = None) -> "Scene":
         hd = C.c_void_p()
-        check(load().fases_scene_shoebox(nx, ny, nz, h, C.byref(hd)))
+        dev = -1 if device is None else int(device)
+        check(load().fases_scene_shoebox_dev(nx, ny, nz, h, dev, C.byref(hd)))
         return cls(hd)
 
     @property

@@ -27,6 +27,42 @@ def f32(x) -> float:
     return float(np.float32(x))
 
 
+def write_tessellated_room(path, X: float, Y: float, Z: float, n: int) -> int:
+    """OBJ of an X x Y x Z [m] shoebox whose six walls (groups west, east, south, north,
+    floor, ceiling) are each split into n x n quads of two triangles: a mesh of 12 n^2
+    triangles with shared vertices, for classification at mesh sizes of real scenes.
+    Returns the triangle count."""
+    V, idx = [], {}
+
+    def vid(p):
+        key = tuple(np.round(p, 12))
+        if key not in idx:
+            idx[key] = len(V)
+            V.append(p)
+        return idx[key]
+    planes = [("west", (0, 0, 0), (0, Y, 0), (0, 0, Z)), ("east", (X, 0, 0), (0, 0, Z), (0, Y, 0)),
+              ("south", (0, 0, 0), (0, 0, Z), (X, 0, 0)), ("north", (0, Y, 0), (X, 0, 0), (0, 0, Z)),
+              ("floor", (0, 0, 0), (X, 0, 0), (0, Y, 0)), ("ceiling", (0, 0, Z), (0, Y, 0), (X, 0, 0))]
+    faces = []
+    for g, o, u, w in planes:
+        o, u, w = (np.array(v, float) for v in (o, u, w))
+        tris = []
+        for a in range(n):
+            for b in range(n):
+                q = [vid(o + u * (a + da) / n + w * (b + db) / n) for da, db in ((0, 0), (1, 0), (1, 1), (0, 1))]
+                tris += [(q[0], q[1], q[2]), (q[0], q[2], q[3])]
+        faces.append((g, tris))
+    with open(path, "w") as f:
+        f.write("o room\n")
+        for v in V:
+            f.write("v %.9g %.9g %.9g\n" % tuple(float(a) for a in v))
+        for g, tris in faces:
+            f.write("g %s\n" % g)
+            for t in tris:
+                f.write("f %d %d %d\n" % tuple(i + 1 for i in t))
+    return sum(len(t) for _, t in faces)
+
+
 def time_step(cfl: float, h: float, c: float) -> float:
     """dt = CFL*h/c evaluated in float32 like the reference (main3D.cu:729-730)."""
     return float(np.float32(np.float32(cfl) * np.float32(h)) / np.float32(c))

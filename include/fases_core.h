@@ -134,6 +134,15 @@ int fases_scene_load_obj(const char *path, float h, fases_scene **out);
  * walls west/east/south/north/floor/ceiling (x-,x+,y-,y+,z-,z+) would give. */
 int fases_scene_shoebox(int nx, int ny, int nz, float h, fases_scene **out);
 
+/* The same two scene builders with the signed distance and material id computed on
+ * HIP device `device` (device < 0: on the host, as above). The result is bit-identical
+ * to the host classification. Replaces the reference CUDA build's GPU classification:
+ * build_sdf_gpu_narrowband (parsing3D.h:644-661, dispatched from build_sdf:896-908) and
+ * build_oid_dense_gpu (parsing3D.h:1078-1139, dispatched from build_oid_dense:1145-1154).
+ * A +X ray crossing more than 4096 triangles is FASES_E_ARG. */
+int fases_scene_load_obj_dev(const char *path, float h, int device, fases_scene **out);
+int fases_scene_shoebox_dev(int nx, int ny, int nz, float h, int device, fases_scene **out);
+
 /* Padded grid dims, number of mesh groups (materials), padded extent [m]. */
 int fases_scene_info(const fases_scene *s, int *Nx, int *Ny, int *Nz, int *n_groups, float *L3);
 
@@ -234,6 +243,12 @@ int fases_tile_config(const fases_ctx *ctx, int *tile_x, int *tile_y, int *tile_
 /* Sizes: padded cells, boundary cells, room (updated) cells, device bytes in use. */
 int fases_info(const fases_ctx *ctx, int64_t *n_cells, int64_t *n_bnd, int64_t *n_upd,
                int64_t *dev_bytes);
+
+/* Number of HIP devices visible to this process (0 without a usable runtime or device;
+   the call itself succeeds). Lets a caller pick the GPU or the host classification
+   before it has a context; the reference's CUDA build makes that choice at compile time
+   (parsing3D.h:1145-1154). */
+int fases_device_count(int *n);
 
 #ifdef __cplusplus
 }

@@ -123,3 +123,16 @@ def test_brute_force_distance_tiny_tetra(tmp_path):
                     checked += 1
     assert checked > 100
     assert (sc.S > 0).sum() > 0
+
+
+def test_device_count_and_gpu_classification_without_device():
+    """fases_device_count always succeeds; asking for the GPU classification where no
+    device is visible is an error (FasesError), never a silent host run."""
+    import ctypes as C
+    from fases_amd._lib import FasesError, load
+    n = C.c_int(-1)
+    assert load().fases_device_count(C.byref(n)) == 0 and n.value >= 0
+    if n.value > 0:
+        pytest.skip("a HIP device is visible: covered by tests/test_gpu_geometry.py")
+    with pytest.raises(FasesError):
+        Scene.shoebox(8, 8, 8, np.float32(0.1), device=0)
