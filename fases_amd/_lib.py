@@ -97,6 +97,7 @@ SIGNATURES = {
     "fases_run": (C.c_int, [C.c_void_p, C.c_int64, _dbl, _dbl, _dbl, _dbl]),
     "fases_info": (C.c_int, [C.c_void_p, _i64, _i64, _i64, _i64]),
     "fases_device_count": (C.c_int, [_P(C.c_int)]),
+    "fases_get_slices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _flt]),
     "fases_tile_config": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
     "fases_batch_run": (C.c_int, [_P(C.c_void_p), C.c_int, _i64, _P(_P(C.c_double)), _P(C.c_double)]),
     "fases_adjoint_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _P(C.c_double)]),

@@ -197,6 +197,49 @@ int run_impl(fases_ctx *ctx, long long nsteps, double *ir, double *vel, double *
 
 }  // namespace
 
+namespace {
+
+// Three axis planes of one field into out = [YZ at x = i0 | XZ at y = j0 | XY at z = k0],
+// each x-fastest over its two axes (the order main3D.cu:1326-1334 copies them in), as
+// float32 (storage value times its power-of-two scale).
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_slices(const S *__restrict__ f, float scale, int Nx, int Ny, int Nz, int i0, int j0, int k0, float *__restrict__ out) {
+    const long long nyz = (long long)Ny * Nz, nxz = (long long)Nx * Nz, nxy = (long long)Nx * Ny;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nyz + nxz + nxy) return;
+    long long c;
+    if (t < nyz) {
+        c = i0 + (long long)Nx * t;                                  // t = j + Ny k
+    } else if (t < nyz + nxz) {
+        const long long q = t - nyz, i = q % Nx, k = q / Nx;
+        c = i + Nx * (j0 + (long long)Ny * k);
+    } else {
+        c = (t - nyz - nxz) + nxy * k0;
+    }
+    out[t] = (float)f[c] * scale;
+}
+template <>
+__global__ void __launch_bounds__(256)
+k_slices<__half>(const __half *__restrict__ f, float scale, int Nx, int Ny, int Nz, int i0, int j0, int k0,
+                 float *__restrict__ out) {
+    const long long nyz = (long long)Ny * Nz, nxz = (long long)Nx * Nz, nxy = (long long)Nx * Ny;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nyz + nxz + nxy) return;
+    long long c;
+    if (t < nyz) {
+        c = i0 + (long long)Nx * t;
+    } else if (t < nyz + nxz) {
+        const long long q = t - nyz, i = q % Nx, k = q / Nx;
+        c = i + Nx * (j0 + (long long)Ny * k);
+    } else {
+        c = (t - nyz - nxz) + nxy * k0;
+    }
+    out[t] = __half2float(f[c]) * scale;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *fases_last_error(const fases_ctx *ctx) {
@@ -579,6 +622,39 @@ int fases_set_fields(fases_ctx *ctx, const double *p, const double *u, const dou
 
 int fases_reset(fases_ctx *ctx) {
     return fases_set_fields(ctx, nullptr, nullptr, nullptr, nullptr);
+}
+
+int fases_get_slices(const fases_ctx *cctx, int i0, int j0, int k0, float *out) {
+    fases_ctx *ctx = const_cast<fases_ctx *>(cctx);
+    if (!ctx || !out) return FASES_E_ARG;
+    if (!ctx->have_geo) return set_error(ctx, FASES_E_STATE, "fases_get_slices: no geometry");
+    const int Nx = ctx->Nx, Ny = ctx->Ny, Nz = ctx->Nz;
+    if (i0 < 0 || i0 >= Nx || j0 < 0 || j0 >= Ny || k0 < 0 || k0 >= Nz)
+        return set_error(ctx, FASES_E_RANGE, "fases_get_slices: slice plane outside the grid");
+    const size_t per = (size_t)Ny * Nz + (size_t)Nx * Nz + (size_t)Nx * Ny;
+    HIPCHK(hipSetDevice(ctx->prm.device));
+    if (ctx->n_slices < 4 * per) {
+        int rc = dalloc(ctx, &ctx->d_slices, 4 * per);
+        if (rc) return rc;
+        ctx->n_slices = 4 * per;
+    }
+    const unsigned nb = (unsigned)((per + 255) / 256);
+    for (int f = 0; f < 4; ++f) {
+        float *dst = ctx->d_slices + (size_t)f * per;
+        if (ctx->esz == 8)
+            hipLaunchKernelGGL(k_slices<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double *)ctx->U[0][f],
+                               1.0f, Nx, Ny, Nz, i0, j0, k0, dst);
+        else if (ctx->esz == 4)
+            hipLaunchKernelGGL(k_slices<float>, dim3(nb), dim3(256), 0, ctx->stream, (const float *)ctx->U[0][f],
+                               1.0f, Nx, Ny, Nz, i0, j0, k0, dst);
+        else
+            hipLaunchKernelGGL(k_slices<__half>, dim3(nb), dim3(256), 0, ctx->stream, (const __half *)ctx->U[0][f],
+                               (float)std::ldexp(1.0, ctx->fexp[f]), Nx, Ny, Nz, i0, j0, k0, dst);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpyAsync(out, ctx->d_slices, 4 * per * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return FASES_OK;
 }
 
 int fases_get_fields(const fases_ctx *cctx, double *p, double *u, double *v, double *w) {

@@ -75,6 +75,8 @@ struct fases_ctx {
     // FP16 storage: field scale exponents (value = stored * 2^fexp[f]) and the reduction
     // buffer of the rescale pass (4 maxima + non-finite flag)
     int fexp[4] = {0, 0, 0, 0};
+    float *d_slices = nullptr;           // animation planes of p, u, v, w (allocated on first use)
+    size_t n_slices = 0;
     unsigned *d_amax = nullptr;
     int tz = 0;                          // z planes per block of the tiled kernel (0: not tuned)
     int nw = 16;                         // waves (rows) per block of the tiled kernel: 16 or 8

@@ -344,6 +344,14 @@ class Solver:
         self._chk(self._lib.fases_get_fields(self._ctx, *[_ptr(a, C.c_double) for a in out]))
         return out
 
+    def get_slices(self, i0: int, j0: int, k0: int):
+        """Animation planes of the current p, u, v, w: a [4, Ny*Nz + Nx*Nz + Nx*Ny] float32
+        array, each row [YZ at x = i0 | XZ at y = j0 | XY at z = k0] (fases_get_slices)."""
+        Nx, Ny, Nz = self.dims
+        out = np.zeros((4, Ny * Nz + Nx * Nz + Nx * Ny), np.float32)
+        self._chk(self._lib.fases_get_slices(self._ctx, int(i0), int(j0), int(k0), _ptr(out, C.c_float)))
+        return out
+
     def reset(self) -> None:
         self._chk(self._lib.fases_reset(self._ctx))
 

@@ -196,6 +196,12 @@ int fases_set_fields(fases_ctx *ctx, const double *p, const double *u, const dou
 /* Copies the current state U^n (padded grid, FP64 on the host). Any pointer may be NULL. */
 int fases_get_fields(const fases_ctx *ctx, double *p, double *u, double *v, double *w);
 
+/* Animation planes of the current fields (main3D.cu:1322-1334, copy_planes): out holds
+ * 4 * (Ny*Nz + Nx*Nz + Nx*Ny) floats, fields p, u, v, w in turn, each as
+ * [YZ plane at x = i0 (index j + Ny k) | XZ at y = j0 (i + Nx k) | XY at z = k0 (i + Nx j)].
+ * Gathered on the device, one copy to the host. A plane outside the grid is FASES_E_RANGE. */
+int fases_get_slices(const fases_ctx *ctx, int i0, int j0, int k0, float *out);
+
 /* Zero fields, ADE and PML states, step counter 0. */
 int fases_reset(fases_ctx *ctx);
 

@@ -200,3 +200,52 @@ def test_cli_keep_other_sources(tmp_path):
     r = run_cli(paths)
     assert r.returncode == 0, r.stderr
     assert not (out / "IR0_5.bin").exists()
+
+
+@pytest.mark.gpu
+def test_cli_animation_frames_match_oracle_fields(tmp_path):
+    """Config lines 19-22 (slice planes at x, y, z metres; 60 frames requested): a frame
+    of pressureX/Y/Z.bin and of the six unit energy-flow files after every anim_step-th
+    step (main3D.cu:1322-1388), header pbin = frames written. The planes of the frame
+    after step n are the FP64 fields after n + 1 steps rounded to float32 (the FP64 path
+    equals the oracle), and the flow direction is sign(p) (u, v, w) / (|(u, v, w)| + 1e-20)."""
+    from oracle import oracle as O
+    paths = make_project(tmp_path, precision=0, extra=("0.4", "0.3", "0.2", "60"))
+    print("[case] fases3D animation frames", flush=True)
+    r = run_cli(paths)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    wl = P.load_project(**paths)
+    Nx, Ny, Nz = wl.dims
+    Nt = wl.nsteps
+    out = Path(paths["config"]).parent.parent / "output"
+    hdr = (out / "header.txt").read_text().split()
+    pbin, anim_step = int(hdr[7]), int(hdr[8])
+    assert anim_step == max(Nt // min(60, Nt), 1)
+    assert pbin == len(range(0, Nt, anim_step))
+    f32 = np.float32
+    i0, j0, k0 = [int(f32(x) / f32(0.02)) + 3 for x in (0.4, 0.3, 0.2)]
+    shapes = {"X": (Nz, Ny), "Y": (Nz, Nx), "Z": (Ny, Nx)}
+    frames = {}
+    for ax, shp in shapes.items():
+        a = np.fromfile(out / ("pressure%s.bin" % ax), np.float32)
+        assert a.size == pbin * shp[0] * shp[1]
+        frames[ax] = a.reshape(pbin, *shp)
+    comps = {"X": ("Iy_X", "Iz_X"), "Y": ("Ix_Y", "Iz_Y"), "Z": ("Ix_Z", "Iy_Z")}
+    dirs = {}
+    for ax, names in comps.items():
+        dirs[ax] = [np.fromfile(out / (n + ".bin"), np.float32).reshape(frames[ax].shape) for n in names]
+    # frame q follows step n = q * anim_step, i.e. n + 1 steps from the start
+    q = pbin // 2
+    n_steps = q * anim_step + 1
+    f = O.run_workload(wl, nsteps=n_steps, want_fields=True)["fields"]
+    p, u, v, w = [np.asarray(x).reshape(Nz, Ny, Nx).astype(np.float32) for x in f]
+    planes = {"X": (lambda a: a[:, :, i0]), "Y": (lambda a: a[:, j0, :]), "Z": (lambda a: a[k0, :, :])}
+    inplane = {"X": (v, w), "Y": (u, w), "Z": (u, v)}
+    for ax, cut in planes.items():
+        assert np.array_equal(frames[ax][q], cut(p)), ax
+        pp, uu, vv, ww = cut(p), cut(u), cut(v), cut(w)
+        sp = np.where(pp >= 0, np.float32(1), np.float32(-1))
+        inv = sp / (np.sqrt(uu * uu + vv * vv + ww * ww) + np.float32(1e-20))
+        for got, comp in zip(dirs[ax], inplane[ax]):
+            np.testing.assert_allclose(got[q], cut(comp) * inv, rtol=1e-6, atol=0)
+    assert np.abs(frames["Z"][q]).max() > 0
