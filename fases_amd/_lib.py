@@ -98,6 +98,10 @@ SIGNATURES = {
     "fases_info": (C.c_int, [C.c_void_p, _i64, _i64, _i64, _i64]),
     "fases_device_count": (C.c_int, [_P(C.c_int)]),
     "fases_get_slices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _flt]),
+    "fases_scene_group_areas": (C.c_int, [C.c_void_p, _flt, _flt]),
+    "fases_air_attenuation": (C.c_int, [C.c_float, C.c_float, C.c_float, _flt, C.c_int, _flt]),
+    "fases_read_mat": (C.c_int, [C.c_char_p, _flt, _P(C.c_int)]),
+    "fases_estimate_t60": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_int, _flt, _flt, _flt, _flt, _flt]),
     "fases_tile_config": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
     "fases_batch_run": (C.c_int, [_P(C.c_void_p), C.c_int, _i64, _P(_P(C.c_double)), _P(C.c_double)]),
     "fases_adjoint_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _P(C.c_double)]),

@@ -6,6 +6,12 @@
 // src.txt/rec.txt main3D.cu:861-906 / 919-953 (cell = (int)(x/h) + 3)
 // waveforms       main3D.cu:963-1058 (PCS, Ricker), amplitude scaling main3D.cu:1063-1110
 // IR/VEL files    main3D.cu:1242-1256, 1397-1414 (little-endian float32, no header)
+// <layer>.mat     octave-band absorption, first 8 floats ('#' comments; 3D-solver/include/
+//                 ism3D.h ism_load_impedances:98-152, missing file -> 0)
+// T60 estimate    Eyring with air absorption (ism3D.h estimate_T60s:735-751); air
+//                 attenuation of ISO 9613-1 (3D-solver/core/functions3D.cuh frO/frN/
+//                 alpha_air/p_atm:15-46, main3D.cu:452-484), evaluated with the same
+//                 float/double mixing as the reference's host build
 
 #include "host.hpp"
 #include "../../include/fases_core.h"
@@ -255,6 +261,81 @@ int fases_write_f32(const char *path, const double *x, int64_t n) {
     const size_t w = n > 0 ? std::fwrite(buf.data(), sizeof(float), (size_t)n, fp) : 0;
     std::fclose(fp);
     if (w != (size_t)n) { set_global_error(std::string("short write: ") + path); return FASES_E_IO; }
+    return FASES_OK;
+}
+
+// Air attenuation m [Np/m] at frequency f of ISO 9613-1 for temperature T [K],
+// atmospheric pressure p_a [Pa] and molar water-vapour concentration hm [%]: relaxation
+// frequencies of oxygen and nitrogen, then the classical + rotational term A and the two
+// vibrational terms, scaled from dB to Np (x 8.686 x 0.115). Each intermediate is
+// rounded to float where the reference stores one.
+static float iso_relax_o2(float p_a, float hm) {
+    return p_a * (24.0 + 4.04 * 1e4 * hm * (0.02 + hm) / (0.391 + hm)) / 101325;
+}
+static float iso_relax_n2(float p_a, float hm, float T) {
+    const float a = p_a * std::sqrt(293.15 / T) / 101325;
+    const float b = 9.0 + 280.0 * hm * std::exp(-4.17 * (std::pow(T / 293.15, -1.0 / 3.0) - 1.0));
+    return a * b;
+}
+static float iso_attenuation(float p_a, float hm, float T, float f) {
+    const float a = 1.84 * 1e-11 * 101325 * std::sqrt(T / 293.15) / p_a;
+    const float b = std::pow(T / 293.15, -5.0 / 2.0);
+    const float fo = iso_relax_o2(p_a, hm), fn = iso_relax_n2(p_a, hm, T);
+    const float co = 0.01275 * std::exp(-2239.1 / T) * (1.0 / (fo + (f * f / fo)));
+    const float cn = 0.10680 * std::exp(-3352.0 / T) * (1.0 / (fn + (f * f / fn)));
+    return 0.115 * 8.686 * f * f * (a + b * (co + cn));
+}
+// Barometric pressure at altitude alt [m] and temperature T [K].
+static float barometric(float alt, float T) {
+    const float M_air = 0.02897, g = 9.8067, R = 8.314;
+    return 101325 * std::exp(-M_air * g * alt / (R * T));
+}
+
+int fases_air_attenuation(float T, float alt, float hm, const float *freq, int n, float *m) {
+    if (n < 0 || (n > 0 && (!freq || !m))) { set_global_error("fases_air_attenuation: bad argument"); return FASES_E_ARG; }
+    const float p_a = barometric(alt, T);
+    for (int b = 0; b < n; ++b) m[b] = iso_attenuation(p_a, hm, T, freq[b]);
+    return FASES_OK;
+}
+
+int fases_read_mat(const char *path, float *alpha, int *n_read) {
+    if (!path || !alpha) { set_global_error("fases_read_mat: null argument"); return FASES_E_ARG; }
+    for (int b = 0; b < 8; ++b) alpha[b] = 0.f;
+    if (n_read) *n_read = 0;
+    std::ifstream f(path);
+    if (!f) { set_global_error(std::string("cannot open material file: ") + path); return FASES_E_IO; }
+    std::string line;
+    int n = 0;
+    while (n < 8 && std::getline(f, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.erase(hash);
+        std::istringstream ss(line);
+        float v;
+        while (n < 8 && (ss >> v)) alpha[n++] = v;
+    }
+    if (n_read) *n_read = n;
+    return FASES_OK;
+}
+
+int fases_estimate_t60(float V, float c, float A_tot, int n, const float *A, const float *alpha, const float *m_air,
+                       float *T60, float *f_schroeder) {
+    if (n < 0 || (n > 0 && (!A || !alpha)) || !m_air || !T60) {
+        set_global_error("fases_estimate_t60: bad argument");
+        return FASES_E_ARG;
+    }
+    // Eyring with the air term: T = 55.25 V / (-c S ln(1 - a_mean) + 4 (2 m) V c), the mean
+    // absorption area-weighted; a band without absorption (a_mean < 1e-6) does not decay
+    const float num = 55.25 * V;
+    for (int b = 0; b < 8; ++b) {
+        float am = 0.0;
+        for (int i = 0; i < n; ++i) am += A[i] * alpha[(size_t)i * 8 + b];
+        am /= A_tot;
+        if (am < 1e-6f) { T60[b] = 1e9f; continue; }
+        const float den = -c * A_tot * std::log(1.0 - am) + 4.0 * (2.0f * m_air[b]) * V * c;
+        T60[b] = num / den;
+    }
+    // Schroeder frequency from the 500 Hz band; 0 for a band that does not decay
+    if (f_schroeder) *f_schroeder = (T60[3] < 1e6f) ? 2000.0f * std::sqrt(T60[3] / V) : 0.0f;
     return FASES_OK;
 }
 

@@ -444,6 +444,26 @@ int fases_scene_get(const fases_scene *s, float *S, uint8_t *oid) {
     return FASES_OK;
 }
 
+int fases_scene_group_areas(const fases_scene *s, float *A, float *A_tot) {
+    if (!s) return FASES_E_ARG;
+    const Scene *sc = reinterpret_cast<const Scene *>(s);
+    const Mesh &m = sc->mesh;
+    std::vector<float> acc(m.groups.size(), 0.f);
+    float tot = 0.f;
+    // half the cross-product norm of each placed triangle, summed in triangle order per
+    // group and in total (parsing3D.h mesh_group_areas:101-114)
+    for (const Triangle &t : m.tris) {
+        const F3 a = placed(m, t.a), b = placed(m, t.b), c = placed(m, t.c);
+        const F3 n = vcross(vsub(b, a), vsub(c, a));
+        const float ar = 0.5f * std::sqrt(vdot(n, n));
+        if (t.group >= 0 && t.group < (int)acc.size()) acc[(size_t)t.group] += ar;
+        tot += ar;
+    }
+    if (A) std::copy(acc.begin(), acc.end(), A);
+    if (A_tot) *A_tot = tot;
+    return FASES_OK;
+}
+
 void fases_scene_free(fases_scene *s) { delete reinterpret_cast<Scene *>(s); }
 
 int fases_boundary_map(const uint8_t *oid, int64_t N, int32_t *bmap, int64_t *nB) {

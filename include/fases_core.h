@@ -152,7 +152,32 @@ int fases_scene_group_name(const fases_scene *s, int g, char *buf, int buflen);
 /* Copies S [Nx*Ny*Nz float] and oid [Nx*Ny*Nz uint8] (either may be NULL). */
 int fases_scene_get(const fases_scene *s, float *S, uint8_t *oid);
 
+/* Surface area per mesh group (A[n_groups], may be NULL) and in total [m^2] of the placed
+ * mesh (parsing3D.h mesh_group_areas:101-114), for the reverberation estimate. */
+int fases_scene_group_areas(const fases_scene *s, float *A, float *A_tot);
+
 void fases_scene_free(fases_scene *s);
+
+/* ------------------------------------------------------------------------- */
+/* Reverberation estimate of the preflight (main3D.cu:452-497)                */
+/* ------------------------------------------------------------------------- */
+
+/* Air attenuation [Np/m] of ISO 9613-1 at n frequencies for temperature T [K], altitude
+ * alt [m] (barometric pressure) and molar water-vapour concentration hm [%]
+ * (functions3D.cuh frO/frN/alpha_air/p_atm:15-46; the reference uses hm = 1.1). */
+int fases_air_attenuation(float T, float alt, float hm, const float *freq, int n, float *m);
+
+/* <layer>.mat: the first 8 octave-band absorption coefficients (63 Hz .. 8 kHz); values
+ * not present are 0 (ism3D.h ism_load_impedances:136-150). A missing file is FASES_E_IO
+ * with alpha all 0 (the reference reads it as all 0). */
+int fases_read_mat(const char *path, float *alpha, int *n_read);
+
+/* Eyring reverberation time per octave band with the air term (ism3D.h
+ * estimate_T60s:735-751): n groups with areas A[n] and absorption alpha[n][8], air
+ * attenuation m_air[8]; a band without absorption gets 1e9. f_schroeder (nullable) =
+ * 2000 sqrt(T60[500 Hz] / V), or 0 where that band does not decay (main3D.cu:482). */
+int fases_estimate_t60(float V, float c, float A_tot, int n, const float *A, const float *alpha, const float *m_air,
+                       float *T60, float *f_schroeder);
 
 /* Compact ADE index: bmap[c] = running count where oid not in {0, 0xFF}, else -1
  * (filterABC.h build_boundary_map:95-108). */

@@ -148,6 +148,11 @@ def ref_lib() -> C.CDLL:
         L.ref_boundary_map.argtypes = [_P(C.c_uint8), C.c_int, _P(C.c_int)]
         L.ref_read_fit.restype = None
         L.ref_read_fit.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int] + [_P(C.c_float)] * 7
+        L.ref_group_areas.restype = C.c_int
+        L.ref_group_areas.argtypes = [C.c_char_p, C.c_float, _P(C.c_float), C.c_int, _P(C.c_float)]
+        L.ref_t60.restype = C.c_int
+        L.ref_t60.argtypes = [C.c_char_p, C.c_float, C.c_char_p, C.c_float, C.c_float, _P(C.c_float),
+                              _P(C.c_float), _P(C.c_float), C.c_int]
         _ref = L
     return _ref
 
@@ -185,3 +190,26 @@ def ref_read_fit(path, m, K, L, nMat):
     ref_lib().ref_read_fit(path.encode(), m, K, L, *[_p(a, C.c_float) for a in (R, A, Z, B, Cc, al, be)])
     del arrs
     return R, A, Z, B, Cc, al, be
+
+
+def ref_group_areas(obj_path: str, h: float):
+    """The reference's mesh_group_areas: (areas per group, total) of the placed mesh."""
+    A = np.zeros(256, np.float32)
+    tot = C.c_float()
+    n = ref_lib().ref_group_areas(obj_path.encode(), h, _p(A, C.c_float), A.size, C.byref(tot))
+    if n < 0:
+        raise RuntimeError("reference OBJ parse failed")
+    return A[:n].copy(), np.float32(tot.value)
+
+
+def ref_t60(obj_path: str, h: float, mat_dir: str, V: float, c: float, m_air):
+    """The reference's estimate_T60s on its own areas and .mat loading: (T60[8], alpha[n, 8])."""
+    m = np.ascontiguousarray(m_air, np.float32)
+    T = np.zeros(8, np.float32)
+    al = np.zeros(256 * 8, np.float32)
+    n = ref_lib().ref_t60(obj_path.encode(), h, mat_dir.encode(), V, c, _p(m, C.c_float), _p(T, C.c_float),
+                          _p(al, C.c_float), 256)
+    if n < 0:
+        raise RuntimeError("reference OBJ parse failed")
+    return T, al[:8 * n].reshape(n, 8).copy()
+
