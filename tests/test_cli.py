@@ -249,3 +249,20 @@ def test_cli_animation_frames_match_oracle_fields(tmp_path):
         for got, comp in zip(dirs[ax], inplane[ax]):
             np.testing.assert_allclose(got[q], cut(comp) * inv, rtol=1e-6, atol=0)
     assert np.abs(frames["Z"][q]).max() > 0
+
+
+@pytest.mark.gpu
+def test_cli_gpu_and_host_classification_give_identical_runs(tmp_path):
+    """The CLI classifies the scene on the GPU by default; --host-geometry on the host.
+    Same S and oid, so the IR files are byte-identical."""
+    paths = make_project(tmp_path, precision=1, t_max=0.002)
+    out = Path(paths["config"]).parent.parent / "output"
+    print("[case] fases3D gpu vs host classification", flush=True)
+    r = run_cli(paths)
+    assert r.returncode == 0, r.stderr
+    gpu = [(out / ("IR%d_0.bin" % q)).read_bytes() for q in range(2)]
+    r = run_cli(paths, "--host-geometry")
+    assert r.returncode == 0, r.stderr
+    host = [(out / ("IR%d_0.bin" % q)).read_bytes() for q in range(2)]
+    assert gpu == host and len(gpu[0]) > 0
+

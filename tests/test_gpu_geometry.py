@@ -153,3 +153,21 @@ def test_bad_device_index_is_an_error():
     from fases_amd._lib import FasesError
     with pytest.raises(FasesError):
         Scene.shoebox(8, 8, 8, np.float32(0.1), device=99)
+
+
+def test_ray_crossing_more_than_the_list_holds_is_an_error(tmp_path):
+    """4100 stacked planes across x (two triangles each, a ray meets one of them): every
+    +X ray through them crosses more than the 4096 abscissae the sign pass holds per
+    row, and the call returns an error (the kernel drops the extra crossings and
+    reports the count; nothing is written past the list)."""
+    from fases_amd._lib import FasesError
+    V, tris = [], []
+    for q in range(4100):
+        x = 0.01 + q * 0.0002
+        b = len(V)
+        V += [(x, 0, 0), (x, 0.3, 0), (x, 0.3, 0.3), (x, 0, 0.3)]
+        tris += [(b, b + 1, b + 2), (b, b + 2, b + 3)]
+    p = tmp_path / "planes.obj"
+    _write_obj(p, V, [("planes", tris)])
+    with pytest.raises(FasesError, match="crosses"):
+        Scene.from_obj(str(p), np.float32(0.05), device=0)
