@@ -26,6 +26,13 @@ namespace {
 // by a factor 3 with no FP64 adjoint, by 6.5e-4 with radius 6 and by 1e-6 with 12.
 constexpr int kAdjointPatch = 12;
 
+// Tiled flux adjoint: waves per block (the inner kAdjRows - 4 rows are outputs) and the
+// z planes per block. 16 rows: one block per CU at 95 VGPRs; 8-row blocks (two per CU,
+// 104 VGPRs) were measured slower (reverse sweep 7.25 s vs 5.89 s on the 192x160x128
+// room over 3000 steps): their halo rows cost more than the second block recovers.
+constexpr int kAdjRows = 16;
+constexpr int kAdjTileZ = 32;
+
 // Device buffers of the adjoint, owned by the context (freed with it).
 struct AdjBufs {
     std::vector<void *> allocs;
@@ -344,8 +351,9 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
     const R kp = (R)(0.5 * P.rho * P.c * P.c), kc = (R)(0.5 * P.c), kr = (R)(0.5 / P.rho);
     // flux adjoint: tiled kernel unless the per-cell one was asked for
     const bool tiled = P.adjoint_variant != 1;
-    const int atz = 32;
-    const dim3 agrid((ctx->Nx - 6 + adjtile::TX - 1) / adjtile::TX, (ctx->Ny - 6 + 11) / 12, (ctx->Nz - 6 + atz - 1) / atz);
+    const int atz = kAdjTileZ;
+    const dim3 agrid((ctx->Nx - 6 + adjtile::TX - 1) / adjtile::TX, (ctx->Ny - 6 + kAdjRows - 5) / (kAdjRows - 4),
+                     (ctx->Nz - 6 + atz - 1) / atz);
     for (int c = A.n_ckpt - 1; c >= 0; --c) {
         const long long n0 = A.step0 + (long long)c * A.every;
         const long long len = std::min<long long>(A.every, A.step0 + nsteps - n0);
@@ -426,15 +434,15 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
                 const AdjK<R> K{kp, kc, kr, (R)(t.beta * P.dt / P.h), (R)t.alpha, (R)t.beta};
                 if (tiled) {
                     if (s == 2)
-                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, false, false, true, 16>), agrid, dim3(64, 16), 0,
+                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, false, false, true, kAdjRows>), agrid, dim3(64, kAdjRows), 0,
                                            ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w,
                                            ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz);
                     else if (s == 1)
-                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, false, true, true, 16>), agrid, dim3(64, 16), 0,
+                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, false, true, true, kAdjRows>), agrid, dim3(64, kAdjRows), 0,
                                            ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w,
                                            ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz);
                     else
-                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, true, false, false, 16>), agrid, dim3(64, 16), 0,
+                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, true, false, false, kAdjRows>), agrid, dim3(64, kAdjRows), 0,
                                            ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w,
                                            ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz);
                 } else if (s == 2)

@@ -324,44 +324,50 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
     auto plane = [&](int k) -> long long { return nxy * (long long)max(0, min(k, Nz - 1)); };
     auto lt = [&](const R *a, long long q) -> R { return upd[q] ? bdh * a[q] : R(0); };
 
-    // halo item of this thread (see the tile shapes above)
-    int h_kind = -1, h_r = 0, h_c = 0;
-    long long h_off = 0;
-    {
-        int t = tid;
+    // halo items of this thread: items tid, tid + 64 NW, ... (see the tile shapes above)
+    constexpr int NITEM = 4 * VX + 4 * NW + 4 * NW + 4 * 64 + 2 * LXA + 2 * NW + 2 * NW + 2 * 64;
+    constexpr int NH = (NITEM + 64 * NW - 1) / (64 * NW);
+    int h_kind[NH], h_r[NH], h_c[NH];
+    long long h_off[NH];
+#pragma unroll
+    for (int hq = 0; hq < NH; ++hq) {
+        int t = tid + hq * 64 * NW;
+        int kind = -1, r = 0, c = 0;
         const int n0 = 4 * VX + 4 * NW, n1 = 4 * NW, n2 = 4 * 64, n3 = 2 * LXA + 2 * NW, n4 = 2 * NW, n5 = 2 * 64;
         const int rA[4] = {0, 1, NW + 2, NW + 3};
         const int cA[4] = {0, 1, VX - 2, VX - 1};
         if (t < n0) {
-            h_kind = 0;
-            if (t < 4 * VX) { h_r = rA[t / VX]; h_c = t % VX; }
-            else { t -= 4 * VX; h_c = cA[t / NW]; h_r = 2 + t % NW; }
+            kind = 0;
+            if (t < 4 * VX) { r = rA[t / VX]; c = t % VX; }
+            else { t -= 4 * VX; c = cA[t / NW]; r = 2 + t % NW; }
         } else if ((t -= n0) < n1) {
-            h_kind = 1; h_c = cA[t / NW]; h_r = t % NW;
+            kind = 1; c = cA[t / NW]; r = t % NW;
         } else if ((t -= n1) < n2) {
-            h_kind = 2; h_r = rA[t / 64]; h_c = t % 64;
+            kind = 2; r = rA[t / 64]; c = t % 64;
         } else if ((t -= n2) < n3) {
-            h_kind = 3;
-            if (t < 2 * LXA) { h_r = (t / LXA) ? NW + 1 : 0; h_c = t % LXA; }
-            else { t -= 2 * LXA; h_c = (t / NW) ? LXA - 1 : 0; h_r = 1 + t % NW; }
+            kind = 3;
+            if (t < 2 * LXA) { r = (t / LXA) ? NW + 1 : 0; c = t % LXA; }
+            else { t -= 2 * LXA; c = (t / NW) ? LXA - 1 : 0; r = 1 + t % NW; }
         } else if ((t -= n3) < n4) {
-            h_kind = 4; h_c = (t / NW) ? LXA - 1 : 0; h_r = t % NW;
+            kind = 4; c = (t / NW) ? LXA - 1 : 0; r = t % NW;
         } else if ((t -= n4) < n5) {
-            h_kind = 5; h_r = (t / 64) ? NW + 1 : 0; h_c = t % 64;
+            kind = 5; r = (t / 64) ? NW + 1 : 0; c = t % 64;
         }
-        if (h_kind >= 0) {
+        long long off = 0;
+        if (kind >= 0) {
             int gx, gy;
-            if (h_kind <= 2) {          // V tiles: vp (r, c) -> (Y0-4+r, X0-4+c); vu rows are wy; vv cols are lanes
-                gx = (h_kind == 2) ? X0 - 2 + h_c : X0 - 4 + h_c;
-                gy = (h_kind == 1) ? Y0 - 2 + h_r : Y0 - 4 + h_r;
+            if (kind <= 2) {            // V tiles: vp (r, c) -> (Y0-4+r, X0-4+c); vu rows are wy; vv cols are lanes
+                gx = (kind == 2) ? X0 - 2 + c : X0 - 4 + c;
+                gy = (kind == 1) ? Y0 - 2 + r : Y0 - 4 + r;
             } else {                    // adjoint tiles: lp (r, c) -> (Y0-3+r, X0-3+c)
-                gx = (h_kind == 5) ? X0 - 2 + h_c : X0 - 3 + h_c;
-                gy = (h_kind == 4) ? Y0 - 2 + h_r : Y0 - 3 + h_r;
+                gx = (kind == 5) ? X0 - 2 + c : X0 - 3 + c;
+                gy = (kind == 4) ? Y0 - 2 + r : Y0 - 3 + r;
             }
             gx = max(0, min(gx, Nx - 1));
             gy = max(0, min(gy, Ny - 1));
-            h_off = gx + (long long)Nx * gy;
+            off = gx + (long long)Nx * gy;
         }
+        h_kind[hq] = kind; h_r[hq] = r; h_c[hq] = c; h_off[hq] = off;
     }
 
     // Plane-relative access of the own column (buffer descriptors: scalar plane base,
@@ -422,15 +428,18 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
             T.lp[wy + 1][lx + 1] = ap[Z1];
             T.lu[wy][lx + 1] = ltp(L1, pk);
             T.lv[wy + 1][lx] = ltp(L2, pk);
-            if (h_kind >= 0) {
-                const long long q = pk + h_off;
-                switch (h_kind) {
-                case 0: T.vp[h_r][h_c] = CV::ld(V.p[q], sc.s[0]); break;
-                case 1: T.vu[h_r][h_c] = CV::ld(V.u[q], sc.s[1]); break;
-                case 2: T.vv[h_r][h_c] = CV::ld(V.v[q], sc.s[2]); break;
-                case 3: T.lp[h_r][h_c] = lt(L0, q); break;
-                case 4: T.lu[h_r][h_c] = lt(L1, q); break;
-                default: T.lv[h_r][h_c] = lt(L2, q); break;
+#pragma unroll
+            for (int hq = 0; hq < NH; ++hq) {
+                if (h_kind[hq] < 0) continue;
+                const long long q = pk + h_off[hq];
+                const int r = h_r[hq], c = h_c[hq];
+                switch (h_kind[hq]) {
+                case 0: T.vp[r][c] = CV::ld(V.p[q], sc.s[0]); break;
+                case 1: T.vu[r][c] = CV::ld(V.u[q], sc.s[1]); break;
+                case 2: T.vv[r][c] = CV::ld(V.v[q], sc.s[2]); break;
+                case 3: T.lp[r][c] = lt(L0, q); break;
+                case 4: T.lu[r][c] = lt(L1, q); break;
+                default: T.lv[r][c] = lt(L2, q); break;
                 }
             }
             __syncthreads();
