@@ -342,6 +342,12 @@ int classify_gpu(int device, int Nx, int Ny, int Nz, float h, const std::vector<
         const hipError_t e_ = (x);                                                          \
         if (e_ != hipSuccess) { err = std::string("GPU classification: ") + hipGetErrorString(e_); return FASES_E_HIP; } \
     } while (0)
+    int prev = -1;
+    GCHK(hipGetDevice(&prev));
+    struct DeviceGuard {              // the caller's current device is restored on return
+        int d;
+        ~DeviceGuard() { if (d >= 0) (void)hipSetDevice(d); }
+    } dg{prev};
     GCHK(hipSetDevice(device));
     const int nt = (int)ta.size();
     const size_t N = (size_t)Nx * Ny * Nz;
