@@ -10,8 +10,11 @@
 // steps of the segment in reverse, each stage in reverse order: boundary + ADE adjoint,
 // source gradients, flux adjoint, probe-adjoint gather.
 //
-// Scope: FP32 or FP64 field storage, no cPML (the adjoint of the PML states is not part
-// of this library; fases_adjoint_forward rejects a context with PML faces).
+// cPML: the checkpoints also hold the PML auxiliary states (the segment recompute needs
+// them); the reverse sweep adds the PML part of the flux adjoint and carries the adjoint
+// of the auxiliary states from step to step (k_pml_flux_adj, k_pml_psi_adj).
+//
+// Scope: FP32 or FP64 field storage.
 
 #include "fvm_adjoint.hpp"
 #include "fvm_ctx.hpp"
@@ -61,6 +64,13 @@ struct AdjBufs {
     AdeState lx[4] = {}, rx1 = {}, rx2 = {};
     double *lD = nullptr, *G = nullptr, *gs = nullptr, *gus = nullptr, *dj = nullptr;
     long long gus_n = 0;
+    // cPML: psi checkpoints [6] per checkpoint, psi adjoints [6] (px, ux, py, vy, pz, wz),
+    // per axis the positions whose flux windows reach the band
+    std::vector<double *> ck_pml[6];
+    double *lpsi[6] = {};
+    int *plist[3] = {};
+    int npos[3] = {};
+    size_t pml_n[3] = {};
     // probe gather CSR
     int nT = 0;
     int *tgt = nullptr, *toff = nullptr, *eb = nullptr;
@@ -158,6 +168,90 @@ int build_probe_csr(fases_ctx *ctx, AdjBufs &A) {
     return FASES_OK;
 }
 
+double *const *pml_psi(fases_ctx *ctx) {
+    static thread_local double *v[6];
+    v[0] = ctx->pml.psi_px; v[1] = ctx->pml.psi_ux; v[2] = ctx->pml.psi_py;
+    v[3] = ctx->pml.psi_vy; v[4] = ctx->pml.psi_pz; v[5] = ctx->pml.psi_wz;
+    return v;
+}
+
+// cPML buffers of the adjoint: psi checkpoints, psi adjoints and, per axis, the list of
+// positions in [3, N-3) within 3 cells of a band cell (the reach of a WENO pair and its
+// two faces), where the PML part of the flux adjoint is non-zero.
+int setup_pml(fases_ctx *ctx, AdjBufs &A) {
+    if (!ctx->pml.on) return FASES_OK;
+    const size_t d2 = 2 * (size_t)ctx->pml.d;
+    A.pml_n[0] = d2 * ctx->Ny * ctx->Nz;
+    A.pml_n[1] = (size_t)ctx->Nx * d2 * ctx->Nz;
+    A.pml_n[2] = (size_t)ctx->Nx * ctx->Ny * d2;
+    int rc;
+    for (int c = 0; c < A.n_ckpt; ++c)
+        for (int a = 0; a < 6; ++a) {
+            double *p;
+            if ((rc = adj_alloc(ctx, A, &p, A.pml_n[a / 2]))) return rc;
+            A.ck_pml[a].push_back(p);
+        }
+    for (int a = 0; a < 6; ++a)
+        if ((rc = adj_alloc(ctx, A, &A.lpsi[a], A.pml_n[a / 2]))) return rc;
+    const int nax[3] = {ctx->Nx, ctx->Ny, ctx->Nz};
+    const int *band[3] = {ctx->pml.x.band, ctx->pml.y.band, ctx->pml.z.band};
+    for (int ax = 0; ax < 3; ++ax) {
+        std::vector<int> bh((size_t)nax[ax]), pl;
+        HIPCHK(hipMemcpy(bh.data(), band[ax], bh.size() * 4, hipMemcpyDeviceToHost));
+        for (int p = NG; p < nax[ax] - NG; ++p) {
+            bool near = false;
+            for (int q = std::max(0, p - 3); q <= std::min(nax[ax] - 1, p + 3) && !near; ++q) near = bh[(size_t)q] >= 0;
+            if (near) pl.push_back(p);
+        }
+        A.npos[ax] = (int)pl.size();
+        if (pl.empty()) continue;
+        if ((rc = adj_alloc(ctx, A, &A.plist[ax], pl.size()))) return rc;
+        HIPCHK(hipMemcpyAsync(A.plist[ax], pl.data(), pl.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));      // pl ends with this iteration
+    }
+    return FASES_OK;
+}
+
+int copy_pml(fases_ctx *ctx, AdjBufs &A, int c, bool save) {
+    if (!ctx->pml.on) return FASES_OK;
+    double *const *psi = pml_psi(ctx);
+    for (int a = 0; a < 6; ++a) {
+        double *ck = A.ck_pml[a][(size_t)c];
+        HIPCHK(hipMemcpyAsync(save ? ck : psi[a], save ? psi[a] : ck, A.pml_n[a / 2] * 8, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    }
+    return FASES_OK;
+}
+
+template <typename S, typename R, bool LAST>
+void launch_pml_adj(fases_ctx *ctx, AdjBufs &A, const Grid &g, const FieldSet<S> &vin, const Scl<R> &sc,
+                    const FieldSet<R> &Lo, const FieldSet<R> &Li, const int *pidx, const double *lsh_out,
+                    double *lsh_in, const AdjK<R> &K, double beta) {
+    const PmlAxis *axs[3] = {&ctx->pml.x, &ctx->pml.y, &ctx->pml.z};
+    const R ih = (R)(1.0 / ctx->prm.h);
+    const int np = A.n_ap;
+    for (int ax = 0; ax < 3; ++ax) {
+        if (A.npos[ax] == 0) continue;
+        const PmlAdjAxis pa{axs[ax]->band, axs[ax]->kc, axs[ax]->b, axs[ax]->a, A.lpsi[2 * ax], A.lpsi[2 * ax + 1],
+                            ctx->pml.d};
+        const long long ncell = ax == 0 ? (long long)A.npos[0] * (g.Ny - 6) * (g.Nz - 6)
+                              : ax == 1 ? (long long)(g.Nx - 6) * A.npos[1] * (g.Nz - 6)
+                                        : (long long)(g.Nx - 6) * (g.Ny - 6) * A.npos[2];
+        const dim3 fg((unsigned)((ncell + 255) / 256)), pg((unsigned)((A.pml_n[ax] + 255) / 256));
+        const R *Lu = ax == 0 ? Lo.u : (ax == 1 ? Lo.v : Lo.w);
+#define FASES_PML_ADJ(AXV)                                                                                          \
+        hipLaunchKernelGGL((k_pml_flux_adj<S, R, AXV, LAST>), fg, dim3(256), 0, ctx->stream, g, vin, sc, Lo.p, Lo.u,  \
+                           Lo.v, Lo.w, ctx->d_upd, Li.p, Li.u, Li.v, Li.w, pidx, lsh_out, lsh_in, np, pa,             \
+                           A.plist[AXV], A.npos[AXV], K, ih);                                                          \
+        hipLaunchKernelGGL((k_pml_psi_adj<R, AXV, LAST>), pg, dim3(256), 0, ctx->stream, g, Lo.p, Lu, ctx->d_upd,     \
+                           pidx, lsh_out, np, pa, beta * ctx->prm.dt)
+        if (ax == 0) { FASES_PML_ADJ(0); }
+        else if (ax == 1) { FASES_PML_ADJ(1); }
+        else { FASES_PML_ADJ(2); }
+#undef FASES_PML_ADJ
+    }
+}
+
 template <typename S> FieldSet<S> fs_of(const std::vector<void *> (&v)[4], size_t q) {
     return FieldSet<S>{(S *)v[0][q], (S *)v[1][q], (S *)v[2][q], (S *)v[3][q]};
 }
@@ -251,13 +345,15 @@ int setup(fases_ctx *ctx, AdjBufs &A, long long nsteps, int every) {
     if ((rc = adj_alloc(ctx, A, &A.G, (size_t)ctx->nB * A.P))) return rc;
     if ((rc = adj_alloc(ctx, A, &A.gs, (size_t)std::max(ctx->n_src, 1)))) return rc;
     if ((rc = adj_alloc(ctx, A, &A.dj, (size_t)std::max(ctx->n_rec, 1) * nsteps))) return rc;
+    if ((rc = setup_pml(ctx, A))) return rc;
     return build_probe_csr(ctx, A);
 }
 
 template <typename S, typename R>
 int forward_impl(fases_ctx *ctx, long long nsteps, int every, double *ir) {
     const long long key[8] = {nsteps, every, ctx->N, ctx->nB, (long long)ctx->K * 1000 + ctx->L, ctx->n_patch,
-                              (long long)ctx->n_src * 100000 + ctx->n_rec, (long long)sizeof(S)};
+                              (long long)ctx->n_src * 100000 + ctx->n_rec,
+                              (long long)sizeof(S) + 100LL * (ctx->pml.on ? ctx->pml.d : 0)};
     AdjBufs *A = static_cast<AdjBufs *>(ctx->adj);
     const bool reuse = A && std::equal(key, key + 8, A->key);
     int rc;
@@ -284,6 +380,7 @@ int forward_impl(fases_ctx *ctx, long long nsteps, int every, double *ir) {
                 HIPCHK(hipMemcpyAsync(A->ck_sh[(size_t)c], ctx->shadow[0], 4 * np * 8, hipMemcpyDeviceToDevice,
                                       ctx->stream));
             if ((rc = copy_ade(ctx, A->ck_x[(size_t)c], ctx->phi[1], ctx->psi1[1], ctx->psi2[1]))) return rc;
+            if ((rc = copy_pml(ctx, *A, c, true))) return rc;
         }
         if ((rc = launch_step<S, R>(ctx, ctx->step, nullptr))) return rc;
         if ((rc = launch_receivers<S>(ctx, nsteps, q))) return rc;
@@ -332,6 +429,8 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
         if ((rc = zero_ade(ctx, x))) return rc;
     HIPCHK(hipMemsetAsync(A.G, 0, (size_t)ctx->nB * A.P * 8, ctx->stream));
     HIPCHK(hipMemsetAsync(A.gs, 0, (size_t)std::max(ctx->n_src, 1) * 8, ctx->stream));
+    for (int a = 0; a < 6; ++a)
+        if (A.lpsi[a]) HIPCHK(hipMemsetAsync(A.lpsi[a], 0, A.pml_n[a / 2] * 8, ctx->stream));
     const long long gus_n = (ctx->n_src > 0) ? ctx->n_samp * (ctx->src_opt == FASES_SRC_SINE ? ctx->n_src : 1) : 0;
     if (grad_us && gus_n > 0 && ctx->src_opt != FASES_SRC_SINE) {
         if (A.gus_n < gus_n) {
@@ -370,6 +469,7 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
             HIPCHK(hipMemcpyAsync(ctx->psi1[1], x.p1, nl, hipMemcpyDeviceToDevice, ctx->stream));
             HIPCHK(hipMemcpyAsync(ctx->psi2[1], x.p2, nl, hipMemcpyDeviceToDevice, ctx->stream));
         }
+        if ((rc = copy_pml(ctx, A, c, false))) return rc;
         // recompute the segment into the slots
         for (long long q = 0; q < len; ++q) {
             if ((rc = copy_ade(ctx, A.xs[(size_t)q], ctx->phi[1], ctx->psi1[1], ctx->psi2[1]))) return rc;
@@ -482,6 +582,14 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
                                            A.ap_part, Li.p, Li.u, Li.v, Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w,
                                            A.lam_sh[iU0], Kd);
                 }
+                if (ctx->pml.on) {
+                    const double *lso = nap ? A.lam_sh[t.lout] : nullptr;
+                    double *lsi = nap ? A.lam_sh[t.lin] : nullptr;
+                    if (s == 2)
+                        launch_pml_adj<S, R, true>(ctx, A, g, t.vin, sc, Lo, Li, pidx, lso, lsi, K, t.beta);
+                    else
+                        launch_pml_adj<S, R, false>(ctx, A, g, t.vin, sc, Lo, Li, pidx, lso, lsi, K, t.beta);
+                }
                 if (ctx->nB > 0)
                     hipLaunchKernelGGL((k_probe_gather<R>), tgrid, dim3(sb), 0, ctx->stream, A.nT, A.tgt, A.toff, A.eb,
                                        A.ew, A.lD, Li.p, Li.u, Li.v, Li.w, pidx, A.lam_sh[t.lin], nap);
@@ -519,8 +627,6 @@ int fases_adjoint_forward(fases_ctx *ctx, int64_t nsteps, int checkpoint_every, 
         return set_error(ctx, FASES_E_STATE, "fases_adjoint_forward: geometry and materials must be set first");
     if (nsteps < 1 || checkpoint_every < 1)
         return set_error(ctx, FASES_E_ARG, "fases_adjoint_forward: nsteps and checkpoint_every must be >= 1");
-    if (ctx->pml.on)
-        return set_error(ctx, FASES_E_ARG, "fases_adjoint_forward: the adjoint has no cPML; disable the PML faces");
     if (ctx->prm.precision == FASES_FP16)
         return set_error(ctx, FASES_E_ARG, "fases_adjoint_forward: FP16 storage has no adjoint; use FP32 or FP64");
     HIPCHK(hipSetDevice(ctx->prm.device));

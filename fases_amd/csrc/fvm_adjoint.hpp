@@ -760,4 +760,139 @@ __global__ void k_receiver_adj(R *Lp, const int *__restrict__ cell, int n_rec, c
     }
 }
 
+
+// cPML adjoint (forward: functions3D.cuh:681-756, pml_apply). An updated cell in the
+// band of axis a adds beta dt (kc_a D_a + psi_a) to p and to the a-velocity, where D_a is
+// that field's axis-a flux difference over h and psi_a the step's auxiliary state, and
+// the last stage advances psi_a <- b psi_a + a_a D_a. The adjoint of D_a therefore carries,
+// beyond the main flux adjoint's seed beta dt / h lout, the extra seed
+//   e = beta dt / h kc_a lout  (+ a_a / h lpsi on the last stage),
+// lpsi being the adjoint of the advanced psi. k_pml_flux_adj adds the axis-a flux VJP of
+// those extra seeds to the stage-input adjoint (the VJP is linear in its seeds, so this
+// sum equals the VJP of the full seed); k_pml_psi_adj then turns lpsi into the adjoint of
+// the step's psi: lpsi <- b lpsi + beta dt lout on the last stage, lpsi += beta dt lout on
+// the others (all three stages read the same psi). The psi values themselves never enter.
+struct PmlAdjAxis {
+    const int *band;           // per position along the axis: -1, or the psi tile slot [0, 2d)
+    const double *kc, *b, *a;
+    double *lpsi_p, *lpsi_u;   // adjoints of psi_p and of the axis velocity's psi, tile layout
+    int d;
+};
+
+// psi tile index of the cell (i, j, k) at band slot bnd of axis AX (fvm_device.hpp layout)
+template <int AX>
+__device__ inline long long pml_tile_index(int bnd, int i, int j, int k, int Nx, int Ny, int d) {
+    if (AX == 0) return bnd + (long long)(2 * d) * (j + (long long)Ny * k);
+    if (AX == 1) return i + (long long)Nx * (bnd + (long long)(2 * d) * k);
+    return i + (long long)Nx * (j + (long long)Ny * bnd);
+}
+
+// Extra-seed flux VJP of axis AX on the cells whose windows reach the band: positions
+// along AX from plist (npos of them), the two other axes over [3, N-3). One thread per
+// cell; adds to the stage-input adjoint (and to its FP64 shadow on adjoint-patch cells).
+template <typename S, typename R, int AX, bool LAST>
+__global__ void __launch_bounds__(256)
+k_pml_flux_adj(Grid g, FieldSet<S> V, Scl<R> sc, const R *__restrict__ L0, const R *__restrict__ L1,
+               const R *__restrict__ L2, const R *__restrict__ L3, const uint8_t *__restrict__ upd, R *lin0, R *lin1,
+               R *lin2, R *lin3, const int *__restrict__ pidx, const double *__restrict__ lsh_out, double *lsh_in,
+               int np, PmlAdjAxis pa, const int *__restrict__ plist, int npos, AdjK<R> K, R ih) {
+    const int ni = AX == 0 ? npos : g.Nx - 6, nj = AX == 1 ? npos : g.Ny - 6, nk = AX == 2 ? npos : g.Nz - 6;
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)ni * nj * nk) return;
+    const int a0 = (int)(t % ni);
+    t /= ni;
+    const int a1 = (int)(t % nj), a2 = (int)(t / nj);
+    const int i = AX == 0 ? plist[a0] : a0 + 3, j = AX == 1 ? plist[a1] : a1 + 3, k = AX == 2 ? plist[a2] : a2 + 3;
+    const long long C = (long long)i + (long long)g.Nx * j + g.nxy * k;
+    const long long s = AX == 0 ? 1LL : (AX == 1 ? (long long)g.Nx : g.nxy);
+    const int pos = AX == 0 ? i : (AX == 1 ? j : k);
+    const int nax = AX == 0 ? g.Nx : (AX == 1 ? g.Ny : g.Nz);
+    constexpr int fu = 1 + AX;
+    const R *Lf[4] = {L0, L1, L2, L3};
+    const S *Vf[4] = {V.p, V.u, V.v, V.w};
+    auto lraw = [&](int f, long long X) -> R {
+        const int w = pidx ? pidx[X] : -1;
+        return w >= 0 ? (R)lsh_out[(long long)f * np + w] : Lf[f][X];
+    };
+    // extra seed of field f (0 = p, 1 = the AX velocity) at the cell X = C + (pX - pos) s
+    auto es = [&](int f, int pX) -> R {
+        const int bnd = pa.band[pX];
+        const long long X = C + (long long)(pX - pos) * s;
+        if (bnd < 0 || !upd[X]) return R(0);
+        R v = K.bdh * (R)pa.kc[pX] * lraw(f ? fu : 0, X);
+        if (LAST) {
+            const int ii = AX == 0 ? pX : i, jj = AX == 1 ? pX : j, kk = AX == 2 ? pX : k;
+            const long long T = pml_tile_index<AX>(bnd, ii, jj, kk, g.Nx, g.Ny, pa.d);
+            v += (R)pa.a[pX] * ih * (R)(f ? pa.lpsi_u[T] : pa.lpsi_p[T]);
+        }
+        return v;
+    };
+    auto vv = [&](int f, long long X) -> R { return Conv<S, R>::ld(Vf[f][X], sc.s[f]); };
+    R ap = R(0), au = R(0);
+#pragma unroll
+    for (int q = -2; q <= 2; ++q) {
+        const int cpos = pos - q;
+        if (cpos < 2 || cpos > nax - 3) continue;
+        const R ep0 = es(0, cpos), eu0 = es(1, cpos);
+        const R ep1 = cpos + 1 <= nax - 1 ? es(0, cpos + 1) : R(0), eu1 = cpos + 1 <= nax - 1 ? es(1, cpos + 1) : R(0);
+        const R epm = es(0, cpos - 1), eum = es(1, cpos - 1);
+        if (ep0 == R(0) && eu0 == R(0) && ep1 == R(0) && eu1 == R(0) && epm == R(0) && eum == R(0)) continue;
+        const long long Cq = C - q * s;
+        R mpL, mpR, muL, muR, tpL, tpR, tuL, tuR;
+        godunov_vjp(ep1 - ep0, eu1 - eu0, K.kp, K.kc, K.kr, mpL, tpR, muL, tuR);
+        godunov_vjp(ep0 - epm, eu0 - eum, K.kp, K.kc, K.kr, tpL, mpR, tuL, muR);
+        R gp[5], gu[5];
+        weno_pair_vjp(vv(0, Cq - 2 * s), vv(0, Cq - s), vv(0, Cq), vv(0, Cq + s), vv(0, Cq + 2 * s), mpL, mpR, gp);
+        weno_pair_vjp(vv(fu, Cq - 2 * s), vv(fu, Cq - s), vv(fu, Cq), vv(fu, Cq + s), vv(fu, Cq + 2 * s), muL, muR,
+                      gu);
+        ap += gp[q + 2];
+        au += gu[q + 2];
+    }
+    if (ap == R(0) && au == R(0)) return;
+    R *li[4] = {lin0, lin1, lin2, lin3};
+    const int w = pidx ? pidx[C] : -1;
+    if (w >= 0) {
+        double *o0 = lsh_in + w, *o1 = lsh_in + (long long)fu * np + w;
+        *o0 += (double)ap;
+        *o1 += (double)au;
+        li[0][C] = (R)*o0;
+        li[fu][C] = (R)*o1;
+        return;
+    }
+    li[0][C] += ap;
+    li[fu][C] += au;
+}
+
+// Adjoint of the step's psi of axis AX, one thread per psi tile entry (see above).
+template <typename R, int AX, bool LAST>
+__global__ void __launch_bounds__(256)
+k_pml_psi_adj(Grid g, const R *__restrict__ Lp, const R *__restrict__ Lu, const uint8_t *__restrict__ upd,
+              const int *__restrict__ pidx, const double *__restrict__ lsh_out, int np, PmlAdjAxis pa, double bdt) {
+    const int d2 = 2 * pa.d;
+    const int n0 = AX == 0 ? d2 : g.Nx, n1 = AX == 1 ? d2 : g.Ny, n2 = AX == 2 ? d2 : g.Nz;
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)n0 * n1 * n2) return;
+    const long long T = t;
+    const int a0 = (int)(t % n0);
+    t /= n0;
+    const int a1 = (int)(t % n1), a2 = (int)(t / n1);
+    const int bnd = AX == 0 ? a0 : (AX == 1 ? a1 : a2);
+    const int nax = AX == 0 ? g.Nx : (AX == 1 ? g.Ny : g.Nz);
+    // the position of this slot: low face NG + bnd, high face nax - NG - d + (bnd - d)
+    const int pos = bnd < pa.d ? NG + bnd : nax - NG - pa.d + (bnd - pa.d);
+    if (pos < 0 || pos >= nax || pa.band[pos] != bnd) return;
+    const int i = AX == 0 ? pos : a0, j = AX == 1 ? pos : a1, k = AX == 2 ? pos : a2;
+    if (i < NG || i >= g.Nx - NG || j < NG || j >= g.Ny - NG || k < NG || k >= g.Nz - NG) return;
+    const long long X = (long long)i + (long long)g.Nx * j + g.nxy * k;
+    double lp = 0.0, lu = 0.0;
+    if (upd[X]) {
+        const int w = pidx ? pidx[X] : -1;
+        lp = w >= 0 ? lsh_out[w] : (double)Lp[X];
+        lu = w >= 0 ? lsh_out[(long long)(1 + AX) * np + w] : (double)Lu[X];
+    }
+    const double b = LAST ? (double)(R)pa.b[pos] : 1.0;
+    pa.lpsi_p[T] = b * pa.lpsi_p[T] + bdt * lp;
+    pa.lpsi_u[T] = b * pa.lpsi_u[T] + bdt * lu;
+}
+
 }  // namespace fasesdev

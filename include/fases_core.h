@@ -239,9 +239,9 @@ int fases_run(fases_ctx *ctx, int64_t nsteps, double *ir, double *vel,
               double *elapsed_ms, double *kernel_ms);
 
 /* Forward run of nsteps steps from the current state that also stores a checkpoint
- * (fields, ADE states, FP64 patch shadows) every checkpoint_every steps for
- * fases_adjoint. ir as in fases_run. FP32 or FP64 storage; contexts with PML faces
- * are rejected (no cPML adjoint). Checkpoint and segment buffers stay allocated in the
+ * (fields, ADE states, cPML auxiliary states, FP64 patch shadows) every
+ * checkpoint_every steps for fases_adjoint. ir as in fases_run. FP32 or FP64 storage
+ * (FP16 is rejected). Checkpoint and segment buffers stay allocated in the
  * context and are reused by the next call of the same shape.
  * No counterpart in the reference (gradients there come from fitting loops outside the
  * solver, fminconFIT.py / vectFIT.py driving repeated runs). */

@@ -41,9 +41,16 @@ def perturbed(m, seed=5, rel=0.2):
     return m2
 
 
+def pml_room(nsteps=150):
+    """The FD room with cPML bands on three faces, one per axis, low and high faces
+    (keys U D R L F B: z+, x+ and y-), 5 cells deep."""
+    wl = room(nsteps)
+    return dataclasses.replace(wl, params=dataclasses.replace(wl.params, pml_keys=(1, 0, 1, 0, 0, 1), pml_cells=5))
+
+
 class Setup:
-    def __init__(self, prec="fp64", nsteps=150):
-        self.wl = room(nsteps)
+    def __init__(self, prec="fp64", nsteps=150, wl=None):
+        self.wl = room(nsteps) if wl is None else wl
         self.sv = W.make_solver(self.wl, prec)
         self.sv.set_materials(perturbed(self.wl.materials))
         self.sv.set_fields(self.wl.initial_pressure())
@@ -196,13 +203,61 @@ def test_autograd_function_and_lbfgs():
     s.sv.close()
 
 
-def test_adjoint_rejects_pml_and_fp16():
+@pytest.fixture(scope="module")
+def spml():
+    s = Setup("fp64", wl=pml_room())
+    yield s
+    s.sv.close()
+
+
+def test_pml_gradients_match_fd(spml):
+    """cPML on z+, x+ and y-: the adjoint carries the PML part of the flux adjoint and the
+    adjoint of the auxiliary states; material and drive gradients against central FD."""
+    announce("adjoint with cPML vs FD")
+    (gm, gs, _), _ = spml.grad(every=64)
+    wl = spml.wl
+    s0 = wl.src_sval[0]
+    h = H_FD * s0
+    fd = (spml.J(sval=[s0 + h])[0] - spml.J(sval=[s0 - h])[0]) / (2 * h)
+    print("drive: fd %.10e adjoint %.10e" % (fd, gs[0]))
+    assert abs(gs[0] - fd) <= 1e-6 * abs(fd)
+    for mi, (name, k) in [(0, ("Rinf", None)), (1, ("A", 0)), (2, ("zeta", 1)), (0, ("B", 0)), (1, ("alpha", 0))]:
+        base = getattr(wl.materials, name)
+        v0 = base[mi] if k is None else base[mi, k]
+        hh = H_FD * abs(v0)
+        vals = []
+        for sg in (1, -1):
+            m2 = copy.deepcopy(wl.materials)
+            arr = getattr(m2, name)
+            if k is None:
+                arr[mi] = v0 + sg * hh
+            else:
+                arr[mi, k] = v0 + sg * hh
+            vals.append(spml.J(m=m2)[0])
+        fdm = (vals[0] - vals[1]) / (2 * hh)
+        pi = FIELDS.index((name, k))
+        print("%s[%d,%s]: fd %.10e adjoint %.10e" % (name, mi, k, fdm, gm[mi, pi]))
+        assert abs(gm[mi, pi] - fdm) <= 1e-4 * abs(fdm) + 1e-7 * np.abs(gm[:3]).max()
+
+
+def test_pml_checkpoint_interval_and_fp32(spml):
+    """The psi checkpoints restore the PML states exactly (gradients independent of the
+    checkpoint interval); FP32 gradients stay close to FP64 with the PML on."""
+    announce("adjoint with cPML: checkpoint interval, fp32 vs fp64")
+    (a, sa, _), _ = spml.grad(every=64)
+    (b, sb, _), _ = spml.grad(every=7)
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+    s32 = Setup("fp32", wl=pml_room())
+    (g32, s32g, _), _ = s32.grad()
+    s32.sv.close()
+    for q in range(3):
+        assert np.abs(g32[q] - a[q]).max() <= 1e-3 * np.abs(a[q]).max()
+    assert abs(s32g[0] - sa[0]) <= 1e-4 * abs(sa[0])
+
+
+def test_adjoint_rejects_fp16():
     announce("adjoint error paths")
     wl = W.ade_room(room=(24, 20, 18), h=0.02, nsteps=10, seed=2, n_rec=1, K=1, L=1, pml_cells=4)
-    sv = W.make_solver(wl, "fp64")
-    with pytest.raises(FasesError, match="PML"):
-        sv.adjoint_forward(10, 4)
-    sv.close()
     wl2 = dataclasses.replace(wl, params=dataclasses.replace(wl.params, pml_keys=(0,) * 6))
     sv = W.make_solver(wl2, "fp16")
     with pytest.raises(FasesError, match="FP16"):
