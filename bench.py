@@ -75,6 +75,18 @@ def geometry_timing(room, h) -> dict:
     return out
 
 
+def chunked_run(sv, steps, what, chunk=1000):
+    """IR blocks of `steps` steps run in chunks, with a progress line per chunk (the
+    FP64 path over thousands of steps of the largest room takes minutes)."""
+    out, done = [], 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        out.append(sv.run(n)[0])
+        done += n
+        log("%s: %d / %d steps" % (what, done, steps))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,7 +154,7 @@ def main() -> int:
     ir_err, ir64 = None, None
     if not args.no_fp64_check and args.precision != "fp64":
         sv64 = W.make_solver(wl, "fp64")
-        ir64 = np.concatenate([sv64.run(args.warmup)[0], sv64.run(args.steps)[0]], axis=1)
+        ir64 = np.concatenate([sv64.run(args.warmup)[0]] + chunked_run(sv64, args.steps, "fp64 check"), axis=1)
         sv64.close()
         ir_err = rel_l2(ir, ir64)
         log("IR rel L2 vs FP64 path over %d steps: %.3e (|IR|max %.3e)" % (total, ir_err, np.abs(ir64).max()))
@@ -152,6 +164,7 @@ def main() -> int:
         sv16 = W.make_solver(wl, "fp16")
         ir16w, _, _, _ = sv16.run(args.warmup)
         ir16t, _, ms16, kms16 = sv16.run(args.steps, kernel_timing=True)
+        log("fp16 storage run done")
         sv16.close()
         ir16 = np.concatenate([ir16w, ir16t], axis=1)
         fp16 = {"value": round(cells * args.steps / (ms16 * 1e-3) / 1e6, 2), "unit": "Mcell-updates/s",
