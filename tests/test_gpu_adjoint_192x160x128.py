@@ -23,8 +23,9 @@ drive gradient does not at this size (measured 3 % to 14 % off on this misfit, 3
 on J = sum ir^2, over 200 to 3000 steps), so it is reported, not asserted. The drive
 gradient is read off the adjoint at the source cell while the wavelet is on (the first
 ~40 steps), after the adjoint has passed through the WENO-Z eps-level tails of the early
-wave front; beyond the FP64 patch (radius 12 around the source) that VJP runs in FP32,
-where its large cancelling terms lose the result."""
+wave front, beyond the FP64 patch (radius 12 around the source). There the FP32-stored
+states differ from FP64's where the WENO-Z Jacobian is most sensitive to them: FP64
+adjoint arithmetic over the same FP32 states was measured just as far off (16 %)."""
 import copy
 import sys
 
