@@ -168,11 +168,13 @@ int build_probe_csr(fases_ctx *ctx, AdjBufs &A) {
     return FASES_OK;
 }
 
-double *const *pml_psi(fases_ctx *ctx) {
-    static thread_local double *v[6];
-    v[0] = ctx->pml.psi_px; v[1] = ctx->pml.psi_ux; v[2] = ctx->pml.psi_py;
-    v[3] = ctx->pml.psi_vy; v[4] = ctx->pml.psi_pz; v[5] = ctx->pml.psi_wz;
-    return v;
+// The context's six psi arrays in the adjoint's order (px, ux, py, vy, pz, wz).
+struct PmlPsi {
+    double *a[6];
+};
+PmlPsi pml_psi(const fases_ctx *ctx) {
+    return PmlPsi{{ctx->pml.psi_px, ctx->pml.psi_ux, ctx->pml.psi_py, ctx->pml.psi_vy, ctx->pml.psi_pz,
+                   ctx->pml.psi_wz}};
 }
 
 // cPML buffers of the adjoint: psi checkpoints, psi adjoints and, per axis, the list of
@@ -214,10 +216,10 @@ int setup_pml(fases_ctx *ctx, AdjBufs &A) {
 
 int copy_pml(fases_ctx *ctx, AdjBufs &A, int c, bool save) {
     if (!ctx->pml.on) return FASES_OK;
-    double *const *psi = pml_psi(ctx);
+    const PmlPsi psi = pml_psi(ctx);
     for (int a = 0; a < 6; ++a) {
         double *ck = A.ck_pml[a][(size_t)c];
-        HIPCHK(hipMemcpyAsync(save ? ck : psi[a], save ? psi[a] : ck, A.pml_n[a / 2] * 8, hipMemcpyDeviceToDevice,
+        HIPCHK(hipMemcpyAsync(save ? ck : psi.a[a], save ? psi.a[a] : ck, A.pml_n[a / 2] * 8, hipMemcpyDeviceToDevice,
                               ctx->stream));
     }
     return FASES_OK;
