@@ -234,7 +234,9 @@ def main() -> int:
                    "checkpoint_and_slot_field_bytes": int(16 * info_a["n_cells"] * (3 * 64 + 1 + -(-wa.nsteps // 64))),
                    "drive_gradient": float(gs_a[0]), "drive_gradient_linear_estimate": lin,
                    "note": "J = sum ir^2; the IR is nearly linear in the drive, so dJ/dsval ~ 2 J / sval; "
-                           "gradient and FD / FP64 agreement are tested in tests/test_gpu_adjoint.py"}
+                           "gradient and FD / FP64 agreement are tested in tests/test_gpu_adjoint.py and, on "
+                           "this room at full length, in tests/test_gpu_adjoint_192x160x128.py (FP64 vs FD; FP32 "
+                           "material gradients vs FP64; the FP32 drive gradient is off by 3-14 % there)"}
         log("adjoint: forward %.0f ms, gradient %.0f ms (x%.2f)" % (fwd_ms, (t_c - t_a) * 1e3,
                                                                   adjoint["gradient_over_forward"]))
 
