@@ -235,8 +235,8 @@ def main() -> int:
                    "drive_gradient": float(gs_a[0]), "drive_gradient_linear_estimate": lin,
                    "note": "J = sum ir^2; the IR is nearly linear in the drive, so dJ/dsval ~ 2 J / sval; "
                            "gradient and FD / FP64 agreement are tested in tests/test_gpu_adjoint.py and, on "
-                           "this room at full length, in tests/test_gpu_adjoint_192x160x128.py (FP64 vs FD; FP32 "
-                           "material gradients vs FP64; the FP32 drive gradient is off by 3-14 % there)"}
+                           "this room at full length, in tests/test_gpu_adjoint_192x160x128.py (FP64 vs FD, FP32 "
+                           "vs FP64); FP64 radius 32 around the source (source_patch)"}
         log("adjoint: forward %.0f ms, gradient %.0f ms (x%.2f)" % (fwd_ms, (t_c - t_a) * 1e3,
                                                                   adjoint["gradient_over_forward"]))
 

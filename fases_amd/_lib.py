@@ -46,6 +46,7 @@ class fases_params(C.Structure):
         ("tile_z", C.c_int),
         ("adjoint_variant", C.c_int),
         ("tile_rows", C.c_int),
+        ("source_patch", C.c_int),
     ]
 
 

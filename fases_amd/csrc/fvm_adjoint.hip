@@ -24,10 +24,6 @@
 
 namespace {
 
-// Chebyshev radius of the FP64 adjoint around point sources (reduced-precision storage),
-// the forward's FP64 radius: measured on a 60x50x44 room, the FP32 drive gradient is off
-// by a factor 3 with no FP64 adjoint, by 6.5e-4 with radius 6 and by 1e-6 with 12.
-constexpr int kAdjointPatch = 12;
 
 // Tiled flux adjoint: waves per block (the inner kAdjRows - 4 rows are outputs) and the
 // z planes per block. 16 rows: one block per CU at 95 VGPRs; 8-row blocks (two per CU,
@@ -57,7 +53,8 @@ struct AdjBufs {
     // the source-patch cells ([4][np] each; reduced-precision storage only)
     void *lam[4][4] = {};
     double *lam_sh[4] = {};
-    // adjoint patch: the forward patch cells within kAdjointPatch of a source
+    // adjoint patch: the forward patch cells within the FP64 source radius
+    // (source_patch_radius: the FP64 adjoint covers the whole forward patch)
     int n_ap = 0;
     int *ap_cells = nullptr, *ap_idx = nullptr;
     double *ap_part = nullptr;                   // pair-VJP partials of the patch, [15][2][n_ap]
@@ -318,14 +315,17 @@ int setup(fases_ctx *ctx, AdjBufs &A, long long nsteps, int every) {
         if (!scell.empty())
             HIPCHK(hipMemcpy(scell.data(), ctx->d_scell, scell.size() * 4, hipMemcpyDeviceToHost));
         const long long nx = ctx->Nx, nxy = (long long)ctx->Nx * ctx->Ny;
+        const long long rad = source_patch_radius(ctx);
         std::vector<int> apc, api((size_t)ctx->N, -1);
         for (int c : ctx->h_patch) {
             const long long i = c % nx, j = (c / nx) % ctx->Ny, k = c / nxy;
             bool near = false;
             for (int s0 : scell) {
                 const long long si = s0 % nx, sj = (s0 / nx) % ctx->Ny, sk = s0 / nxy;
-                if (std::llabs(i - si) <= kAdjointPatch && std::llabs(j - sj) <= kAdjointPatch &&
-                    std::llabs(k - sk) <= kAdjointPatch) { near = true; break; }
+                if (std::llabs(i - si) <= rad && std::llabs(j - sj) <= rad && std::llabs(k - sk) <= rad) {
+                    near = true;
+                    break;
+                }
             }
             if (near) { api[(size_t)c] = (int)apc.size(); apc.push_back(c); }
         }

@@ -23,7 +23,6 @@
 
 namespace {
 
-constexpr int kSourcePatch = 12;  // FP64 shadow radius around point sources (reduced storage)
 
 // cPML profile of one axis (functions3D.cuh:681-756) in FP64: depth x in [0,1] from the
 // interior edge (3 + d, or N-4-d) to the outermost updated cell, sigma = sigma_max x^2,
@@ -257,6 +256,8 @@ int fases_create(const fases_params *p, fases_ctx **out) {
     if (p->pml_cells < 1) return set_error(nullptr, FASES_E_ARG, "fases_create: pml_cells must be >= 1");
     if (p->tile_rows != 0 && p->tile_rows != 8 && p->tile_rows != 16)
         return set_error(nullptr, FASES_E_ARG, "fases_create: tile_rows must be 0, 8 or 16");
+    if (p->source_patch < 0 || p->source_patch > 64)
+        return set_error(nullptr, FASES_E_ARG, "fases_create: source_patch must be in [0, 64]");
     if (p->tile_z < 0) return set_error(nullptr, FASES_E_ARG, "fases_create: tile_z must be >= 0");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
@@ -475,13 +476,15 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
         else { mc.push_back((int)cell[s]); ms.push_back(sval[s]); mf.push_back(freq ? freq[s] : 0.0); }
     }
     int rc;
-    // Reduced-precision storage: cells within kSourcePatch (Chebyshev) of a source keep
-    // FP64 shadows of all three field sets and are updated in FP64 arithmetic. Inside a
+    // Reduced-precision storage: cells within the source radius (Chebyshev, default 12,
+    // fases_params.source_patch) of a source keep FP64 shadows of all three field sets and
+    // are updated in FP64 arithmetic. Inside a
     // step the source cell swings by the full per-stage injection (far above the local
     // pressure) and the RK combination cancels it; FP32 storage or arithmetic in that
     // neighbourhood seeds errors that propagate to every receiver (measured: IR rel L2
     // 1.7e-4 far from the source with no patch, 2e-6 with radius 12).
     if (ctx->prm.precision != FASES_FP64) {
+        const int rad = source_patch_radius(ctx);
         const int Nx = ctx->Nx, Ny = ctx->Ny, Nz = ctx->Nz;
         const long long nxy = (long long)Nx * Ny;
         for (auto &v : ctx->h_upd) if (v == 2) v = 1;
@@ -490,9 +493,9 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
         for (int c0 : ctx->h_bcell) is_bnd[(size_t)c0] = 1;
         for (int c0 : mc) {
             const int si = c0 % Nx, sj = (int)((c0 / Nx) % Ny), sk = (int)(c0 / nxy);
-            for (int k = std::max(0, sk - kSourcePatch); k <= std::min(Nz - 1, sk + kSourcePatch); ++k)
-                for (int j = std::max(0, sj - kSourcePatch); j <= std::min(Ny - 1, sj + kSourcePatch); ++j)
-                    for (int i = std::max(0, si - kSourcePatch); i <= std::min(Nx - 1, si + kSourcePatch); ++i) {
+            for (int k = std::max(0, sk - rad); k <= std::min(Nz - 1, sk + rad); ++k)
+                for (int j = std::max(0, sj - rad); j <= std::min(Ny - 1, sj + rad); ++j)
+                    for (int i = std::max(0, si - rad); i <= std::min(Nx - 1, si + rad); ++i) {
                         const long long C = i + (long long)Nx * j + nxy * k;
                         // boundary cells stay out: the boundary kernel blends their stored values
                         if (ctx->h_upd[(size_t)C] == 1 && !is_bnd[(size_t)C]) {

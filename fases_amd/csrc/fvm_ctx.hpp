@@ -369,6 +369,13 @@ int launch_receivers(fases_ctx *ctx, long long stride, long long n) {
 // FASES_E_NONFINITE when a stored value has overflowed.
 // Per-run set-up shared by the single-context and batched loops: sine sample tables
 // long enough for the run, a waveform-length check, receiver buffers of nsteps rows.
+// Chebyshev radius of the FP64 region around point sources under reduced-precision
+// storage (forward shadows and adjoint): fases_params.source_patch, 12 when 0.
+constexpr int kDefaultSourcePatch = 12;
+inline int source_patch_radius(const fases_ctx *ctx) {
+    return ctx->prm.source_patch > 0 ? ctx->prm.source_patch : kDefaultSourcePatch;
+}
+
 inline int prepare_run(fases_ctx *ctx, long long nsteps) {
     if (ctx->n_src > 0 && ctx->src_opt == FASES_SRC_SINE && 2 * (ctx->step + nsteps) + 1 > ctx->n_samp) {
         // per-source sine samples sin(2 pi f nn dt/2), tabulated on the host (same libm

@@ -259,6 +259,7 @@ class SolverParams:
     tile_z: int = 0                     # tiled kernel z planes per block; 0 = autotune
     adjoint_variant: int = 0            # flux adjoint: 0 tiled, 1 per cell
     tile_rows: int = 0                  # tiled kernel waves per block (8 or 16); 0 = autotune
+    source_patch: int = 0               # FP64 radius around sources under FP32/FP16 storage; 0 = 12
 
     def to_c(self) -> _lib.fases_params:
         p = _lib.fases_params()
@@ -278,6 +279,7 @@ class SolverParams:
         p.tile_z = int(self.tile_z)
         p.adjoint_variant = int(self.adjoint_variant)
         p.tile_rows = int(self.tile_rows)
+        p.source_patch = int(self.source_patch)
         return p
 
 

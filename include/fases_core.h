@@ -77,6 +77,10 @@ typedef struct fases_params {
                                once), 1 per cell (neighbours' pairs re-evaluated)        */
     int    tile_rows;       /* tiled kernel: waves (grid rows) per block, 8 or 16; 0 = autotune
                                together with tile_z                                        */
+    int    source_patch;    /* FP32/FP16 storage: Chebyshev radius of the FP64 region around
+                               each point source (forward and adjoint); 0 = 12. Drive
+                               gradients of large rooms need ~32 (FP32 drive gradient of
+                               the 192x160x128 room: 14 % off FP64 at 12, 8e-4 at 32)     */
 } fases_params;
 
 /* Fills *p with the reference defaults (FP64, device 0, PML depth 20, ...). */
