@@ -57,7 +57,11 @@ struct AdjBufs {
     // (source_patch_radius: the FP64 adjoint covers the whole forward patch)
     int n_ap = 0;
     int *ap_cells = nullptr, *ap_idx = nullptr;
-    double *ap_part = nullptr;                   // pair-VJP partials of the patch, [15][2][n_ap]
+    // the patch dilated by two cells along each axis (every cell whose pair windows reach a
+    // patch cell), grid -> dilated index, and its pair VJPs [3][nD][10]
+    int nD = 0;
+    int *d_cells = nullptr, *d_map = nullptr;
+    double *pv = nullptr;
     AdeState lx[4] = {}, rx1 = {}, rx2 = {};
     double *lD = nullptr, *G = nullptr, *gs = nullptr, *gus = nullptr, *dj = nullptr;
     long long gus_n = 0;
@@ -338,7 +342,28 @@ int setup(fases_ctx *ctx, AdjBufs &A, long long nsteps, int every) {
         HIPCHK(hipStreamSynchronize(ctx->stream));      // apc / api end with this block
         for (auto &l : A.lam_sh)
             if ((rc = adj_alloc(ctx, A, &l, 4 * (size_t)std::max(A.n_ap, 1)))) return rc;
-        if ((rc = adj_alloc(ctx, A, &A.ap_part, 30 * (size_t)std::max(A.n_ap, 1)))) return rc;
+        {
+            const int nax[3] = {ctx->Nx, ctx->Ny, ctx->Nz};
+            const long long st[3] = {1, nx, nxy};
+            std::vector<int> dc, dm((size_t)ctx->N, -1);
+            for (int c : apc) {
+                const int pos[3] = {(int)(c % nx), (int)((c / nx) % ctx->Ny), (int)(c / nxy)};
+                for (int ax = 0; ax < 3; ++ax)
+                    for (int q = -2; q <= 2; ++q) {
+                        const int cpos = pos[ax] - q;
+                        if (cpos < 2 || cpos > nax[ax] - 3) continue;
+                        const long long X = c - q * st[ax];
+                        if (dm[(size_t)X] < 0) { dm[(size_t)X] = (int)dc.size(); dc.push_back((int)X); }
+                    }
+            }
+            A.nD = (int)dc.size();
+            if ((rc = adj_alloc(ctx, A, &A.d_cells, dc.size()))) return rc;
+            if ((rc = adj_alloc(ctx, A, &A.d_map, dm.size()))) return rc;
+            if ((rc = adj_alloc(ctx, A, &A.pv, 30 * (size_t)std::max(A.nD, 1)))) return rc;
+            HIPCHK(hipMemcpyAsync(A.d_cells, dc.data(), dc.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(A.d_map, dm.data(), dm.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));  // dc / dm end with this block
+        }
     }
     for (auto &x : A.lx)
         if ((rc = alloc_ade(ctx, A, x))) return rc;
@@ -563,26 +588,26 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
                     const AdjK<double> Kd{0.5 * P.rho * P.c * P.c, 0.5 * P.c, 0.5 / P.rho, t.beta * P.dt / P.h,
                                           t.alpha, t.beta};
                     const dim3 pg((unsigned)((nap + 63) / 64));
-                    const dim3 qg((unsigned)((15LL * nap + 63) / 64));
+                    const dim3 qg((unsigned)((3LL * A.nD + 63) / 64));
                     const PatchAdjIn<S> pin{t.vin, t.vsh, ctx->d_pidx, (int)np, scd};
                     hipLaunchKernelGGL((k_flux_adj_patch_part<S, R>), qg, dim3(64), 0, ctx->stream, g, pin, Lo.p,
-                                       Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], ctx->d_upd, A.ap_cells, A.ap_idx, nap, Kd,
-                                       A.ap_part);
+                                       Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], ctx->d_upd, A.d_cells, A.nD, A.ap_idx, nap,
+                                       Kd, A.pv);
                     if (s == 2)
                         hipLaunchKernelGGL((k_flux_adj_patch<S, R, false, false, true>), pg, dim3(64), 0, ctx->stream,
-                                           Lo.p, Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], A.ap_cells, A.ap_idx, nap,
-                                           A.ap_part, Li.p, Li.u, Li.v, Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w,
-                                           A.lam_sh[iU0], Kd);
+                                           g, Lo.p, Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], A.ap_cells, A.ap_idx, nap,
+                                           A.d_map, A.nD, A.pv, Li.p, Li.u, Li.v, Li.w, A.lam_sh[t.lin], Lb.p, Lb.u,
+                                           Lb.v, Lb.w, A.lam_sh[iU0], Kd);
                     else if (s == 1)
                         hipLaunchKernelGGL((k_flux_adj_patch<S, R, false, true, true>), pg, dim3(64), 0, ctx->stream,
-                                           Lo.p, Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], A.ap_cells, A.ap_idx, nap,
-                                           A.ap_part, Li.p, Li.u, Li.v, Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w,
-                                           A.lam_sh[iU0], Kd);
+                                           g, Lo.p, Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], A.ap_cells, A.ap_idx, nap,
+                                           A.d_map, A.nD, A.pv, Li.p, Li.u, Li.v, Li.w, A.lam_sh[t.lin], Lb.p, Lb.u,
+                                           Lb.v, Lb.w, A.lam_sh[iU0], Kd);
                     else
                         hipLaunchKernelGGL((k_flux_adj_patch<S, R, true, false, false>), pg, dim3(64), 0, ctx->stream,
-                                           Lo.p, Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], A.ap_cells, A.ap_idx, nap,
-                                           A.ap_part, Li.p, Li.u, Li.v, Li.w, A.lam_sh[t.lin], Lb.p, Lb.u, Lb.v, Lb.w,
-                                           A.lam_sh[iU0], Kd);
+                                           g, Lo.p, Lo.u, Lo.v, Lo.w, A.lam_sh[t.lout], A.ap_cells, A.ap_idx, nap,
+                                           A.d_map, A.nD, A.pv, Li.p, Li.u, Li.v, Li.w, A.lam_sh[t.lin], Lb.p, Lb.u,
+                                           Lb.v, Lb.w, A.lam_sh[iU0], Kd);
                 }
                 if (ctx->pml.on) {
                     const double *lso = nap ? A.lam_sh[t.lout] : nullptr;

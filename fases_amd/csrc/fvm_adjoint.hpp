@@ -176,11 +176,11 @@ k_flux_adj_cell(Grid g, FieldSet<S> V, Scl<R> sc, const R *__restrict__ L0, cons
 // states from the stage input's shadow (forward patch map vidx) and the output adjoint
 // from its FP64 adjoint shadow (adjoint patch map pidx; stored copies elsewhere); writes
 // the FP64 adjoint shadows of the input (and base) set and their stored copies, after
-// the grid kernel has run. Two launches: k_flux_adj_patch_part evaluates, for every patch
-// cell C and every (axis, q) of its 15 window positions, the pair VJPs of the cell
-// C - q s at C's slot (one thread each, 15 np threads: the per-cell sum alone leaves the
-// GPU nearly empty); k_flux_adj_patch adds them up in the per-cell order (bit-identical
-// to a single pass) and writes the results.
+// the grid kernel has run. Two launches: k_flux_adj_patch_part evaluates, once per cell X
+// of the patch dilated by two cells along each axis and per axis, both pair VJPs of X
+// (all five window slots, p and the axis velocity) into pv; k_flux_adj_patch then adds,
+// for every patch cell C, the slot of C in the pairs of C - q s over the axes and q in the
+// per-cell order (bit-identical to a single pass that re-evaluates the pairs).
 template <typename S>
 struct PatchAdjIn {
     FieldSet<S> V;                 // stage input (stored)
@@ -190,18 +190,18 @@ struct PatchAdjIn {
     Scl<double> sc;
 };
 
+// pv layout: [(ax * nD + d) * 10 + m], m < 5 the p-pair window slots, m >= 5 the velocity's.
 template <typename S, typename R>
 __global__ void __launch_bounds__(64)
 k_flux_adj_patch_part(Grid g, PatchAdjIn<S> in, const R *__restrict__ L0, const R *__restrict__ L1,
                       const R *__restrict__ L2, const R *__restrict__ L3, const double *__restrict__ lsh_out,
-                      const uint8_t *__restrict__ upd, const int *__restrict__ cells, const int *__restrict__ pidx,
-                      int np, AdjK<double> K, double *__restrict__ part) {
+                      const uint8_t *__restrict__ upd, const int *__restrict__ dcells, int nD,
+                      const int *__restrict__ pidx, int np, AdjK<double> K, double *__restrict__ pv) {
     const long long tt = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (tt >= 15LL * np) return;
-    const int t = (int)(tt % np), combo = (int)(tt / np);     // combo = 5 ax + (q + 2)
-    const int ax = combo / 5, q = combo % 5 - 2;
-    const int C = cells[t];
-    const int pos[3] = {C % g.Nx, (C / g.Nx) % g.Ny, (int)(C / g.nxy)};
+    if (tt >= 3LL * nD) return;
+    const int d = (int)(tt % nD), ax = (int)(tt / nD);
+    const int X = dcells[d];
+    const int pos[3] = {X % g.Nx, (X / g.Nx) % g.Ny, (int)(X / g.nxy)};
     const int nax[3] = {g.Nx, g.Ny, g.Nz};
     const long long s = ax == 0 ? 1LL : (ax == 1 ? (long long)g.Nx : g.nxy);
     const int fu = 1 + ax;
@@ -216,36 +216,39 @@ k_flux_adj_patch_part(Grid g, PatchAdjIn<S> in, const R *__restrict__ L0, const 
         const int w = in.vidx[x];
         return w >= 0 ? in.vsh[(long long)f * in.nv + w] : Conv<S, double>::ld(Vf[f][x], in.sc.s[f]);
     };
-    double cp = 0.0, cu = 0.0;
-    const int cpos = pos[ax] - q;
-    if (cpos >= 2 && cpos <= nax[ax] - 3) {
-        const long long Cq = C - q * s;
+    double gp[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gu[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (pos[ax] >= 2 && pos[ax] <= nax[ax] - 3) {
+        const long long Cq = X;
         const double lp0 = lt(0, Cq), lu0 = lt(fu, Cq);
         const double mFpP = lt(0, Cq + s) - lp0, mFuP = lt(fu, Cq + s) - lu0;
         const double mFpM = lp0 - lt(0, Cq - s), mFuM = lu0 - lt(fu, Cq - s);
         double mpL, mpR, muL, muR, tpL, tpR, tuL, tuR;
         godunov_vjp(mFpP, mFuP, K.kp, K.kc, K.kr, mpL, tpR, muL, tuR);
         godunov_vjp(mFpM, mFuM, K.kp, K.kc, K.kr, tpL, mpR, tuL, muR);
-        double gp[5], gu[5];
         weno_pair_vjp(vv(0, Cq - 2 * s), vv(0, Cq - s), vv(0, Cq), vv(0, Cq + s), vv(0, Cq + 2 * s), mpL, mpR, gp);
         weno_pair_vjp(vv(fu, Cq - 2 * s), vv(fu, Cq - s), vv(fu, Cq), vv(fu, Cq + s), vv(fu, Cq + 2 * s), muL, muR,
                       gu);
-        cp = gp[q + 2];
-        cu = gu[q + 2];
     }
-    part[(2LL * combo) * np + t] = cp;
-    part[(2LL * combo + 1) * np + t] = cu;
+    double *o = pv + ((long long)ax * nD + d) * 10;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        o[m] = gp[m];
+        o[5 + m] = gu[m];
+    }
 }
 
 template <typename S, typename R, bool ACC_IN, bool ACC_BASE, bool HAS_BASE>
 __global__ void __launch_bounds__(64)
-k_flux_adj_patch(const R *__restrict__ L0, const R *__restrict__ L1, const R *__restrict__ L2,
+k_flux_adj_patch(Grid g, const R *__restrict__ L0, const R *__restrict__ L1, const R *__restrict__ L2,
                  const R *__restrict__ L3, const double *__restrict__ lsh_out, const int *__restrict__ cells,
-                 const int *__restrict__ pidx, int np, const double *__restrict__ part, R *lin0, R *lin1, R *lin2,
-                 R *lin3, double *lsh_in, R *lb0, R *lb1, R *lb2, R *lb3, double *lsh_base, AdjK<double> K) {
+                 const int *__restrict__ pidx, int np, const int *__restrict__ dmap, int nD,
+                 const double *__restrict__ pv, R *lin0, R *lin1, R *lin2, R *lin3, double *lsh_in, R *lb0, R *lb1,
+                 R *lb2, R *lb3, double *lsh_base, AdjK<double> K) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= np) return;
     const int C = cells[t];
+    const int pos[3] = {C % g.Nx, (C / g.Nx) % g.Ny, (int)(C / g.nxy)};
+    const int nax[3] = {g.Nx, g.Ny, g.Nz};
     const R *Lf[4] = {L0, L1, L2, L3};
     auto lraw = [&](int f, long long x) -> double {
         const int w = pidx[x];
@@ -256,8 +259,17 @@ k_flux_adj_patch(const R *__restrict__ L0, const R *__restrict__ L1, const R *__
     for (int f = 0; f < 4; ++f) acc[f] = K.beta * lraw(f, C);
 #pragma unroll
     for (int combo = 0; combo < 15; ++combo) {       // ax-major, q = -2..2: the per-cell order
-        acc[0] += part[(2LL * combo) * np + t];
-        acc[1 + combo / 5] += part[(2LL * combo + 1) * np + t];
+        const int ax = combo / 5, q = combo % 5 - 2;
+        double cp = 0.0, cu = 0.0;
+        const int cpos = pos[ax] - q;
+        if (cpos >= 2 && cpos <= nax[ax] - 3) {
+            const long long s = ax == 0 ? 1LL : (ax == 1 ? (long long)g.Nx : g.nxy);
+            const double *o = pv + ((long long)ax * nD + dmap[C - q * s]) * 10;
+            cp = o[q + 2];
+            cu = o[5 + q + 2];
+        }
+        acc[0] += cp;
+        acc[1 + ax] += cu;
     }
     R *li[4] = {lin0, lin1, lin2, lin3};
 #pragma unroll
