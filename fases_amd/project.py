@@ -87,7 +87,8 @@ def load_project(scene: str, config: str, src: str, rec: str) -> Workload:
     ijk = lambda cidx: (int(cidx % Nx), int((cidx // Nx) % Ny), int(cidx // (Nx * Ny)))
     prm = SolverParams(h=float(h), dt=float(dt), c=float(c), rho=float(rho), precision=prec,
                        pml_keys=tuple(float(v) for v in conf[6:12]),
-                       pml_cells=int(ext.get("pml_cells", 20)), pml_sigma_max=float(ext.get("pml_sigma_max", 0.0)))
+                       pml_cells=int(ext.get("pml_cells", 20)), pml_sigma_max=float(ext.get("pml_sigma_max", 0.0)),
+                       source_patch=int(ext.get("source_patch", 0)))
     wl = Workload(name=Path(scene).stem, room=tuple(d - 6 for d in sc.dims), h=float(h), nsteps=Nt, params=prm,
                   materials=mats, rec_ijk=[ijk(r) for r in rcells], src_ijk=[ijk(s) for s in cells],
                   src_sval=list(sval), src_opt=src_opt, u_s=u_s, src_freq=freq)

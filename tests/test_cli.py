@@ -170,10 +170,13 @@ def test_cli_fp64_ir_files_match_oracle(tmp_path, src_opt):
 
 
 @pytest.mark.gpu
-def test_cli_fp32_default_precision_within_tolerance(tmp_path):
+@pytest.mark.parametrize("extra", [(), ("@source_patch 20",)])
+def test_cli_fp32_default_precision_within_tolerance(tmp_path, extra):
+    """FP32 CLI run against the FP64 oracle; also with a wider FP64 source region set from
+    config.txt (@source_patch)."""
     from oracle import oracle as O
-    paths = make_project(tmp_path, precision=1)
-    print("[case] fases3D fp32", flush=True)
+    paths = make_project(tmp_path, precision=1, extra=extra)
+    print("[case] fases3D fp32 %s" % (extra,), flush=True)
     r = run_cli(paths, "--fp64-ir")
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     wl = P.load_project(**paths)
