@@ -226,8 +226,9 @@ def main() -> int:
         info_a = sa.info()
         sa.close()
         lin = 2.0 * float((ir_c ** 2).sum()) / wa.src_sval[0]
-        adjoint = {"workload": "192x160x128 room dx 0.02, 3 materials (K=4, L=1), Ricker source, 8 receivers, FP32, "
-                               "%d steps, checkpoints every 64 steps" % wa.nsteps,
+        adjoint = {"workload": "192x160x128 room dx 0.02, 3 materials (K=4, L=1), Ricker source, 8 receivers, FP32 "
+                               "(FP64 within radius %d of the source), %d steps, checkpoints every 64 steps"
+                               % (wa.params.source_patch, wa.nsteps),
                    "forward_ms": round(fwd_ms, 1), "checkpointed_forward_ms": round((t_b - t_a) * 1e3, 1),
                    "reverse_ms": round((t_c - t_b) * 1e3, 1),
                    "gradient_over_forward": round((t_c - t_a) * 1e3 / fwd_ms, 3),
