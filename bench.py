@@ -7,8 +7,10 @@ full SSP-RK3 step (3 flux stages + sources + immersed boundary) over the room.
 
 Prints ONE JSON line: Mcell-updates/s (room cells x steps / HIP-event time of the
 step loop), the flux-stage kernel's achieved algorithmic HBM bandwidth vs the 8 TB/s
-peak, the IR relative L2 of this run vs the FP64 path on the same room and window,
-and the FP64 CPU oracle timed on a bounded sample on this host. Beside the headline:
+peak (and vs a streaming copy measured in the same run), its VALU issue rate, the IR
+relative L2 vs the FP64 path over a fixed window that holds the direct sound at every
+receiver (independent of --steps), and the FP64 CPU oracle timed on bounded samples on
+this host. Beside the headline:
 the same room with FP16 storage (per-field power-of-two scales, FP32 math) and its IR
 error vs the FP64 and FP32 runs, and the batched benchmark (32 seeded rooms of 0.9-7.9 M
 cells, 1500-4000 steps each, FP32) run as one batch vs the same jobs run one after the
@@ -29,6 +31,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# non-packed FP32 VALU issue: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (lane-ops/s)
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
+# an IR whose FP64 reference stays below this [Pa] has no signal to compare yet
+IR_FLOOR_PA = 1e-6
 METRIC = "Mcell-updates/s + % HBM roofline; batched-source & adjoint vs plain, same GPU"
 ESZ = {"fp64": 8, "fp32": 4, "fp16": 2}
 DTYPE = {"fp64": "f64", "fp32": "f32", "fp16": "f16-storage/f32-math"}
@@ -75,6 +81,35 @@ def geometry_timing(room, h) -> dict:
     return out
 
 
+def accuracy_window(wl) -> tuple:
+    """Steps until the direct sound has passed every receiver: the longest source-receiver
+    distance at 0.5 cells per step (CFL 0.5) plus three times the wavelet's peak delay,
+    rounded up to 100 steps, at least 600. Returns (steps, last arrival step)."""
+    s = np.asarray(wl.src_ijk[0], float)
+    r = np.asarray(wl.rec_ijk, float)
+    arrive = float(np.sqrt(((r - s) ** 2).sum(axis=1)).max()) / (wl.params.c * wl.params.dt / wl.h)
+    peak = float(np.argmax(np.abs(wl.u_s))) / 2.0       # u_s is sampled at dt/2
+    return max(600, int(np.ceil((arrive + 3.0 * peak) / 100.0)) * 100 + 100), int(round(arrive))
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def profile_json(name: str):
+    p = ROOT / "profiles" / name
+    try:
+        return json.loads(p.read_text()) if p.exists() else None
+    except Exception:
+        return None
+
+
 def chunked_run(sv, steps, what, chunk=1000):
     """IR blocks of `steps` steps run in chunks, with a progress line per chunk (the
     FP64 path over thousands of steps of the largest room takes minutes)."""
@@ -95,6 +130,9 @@ def main() -> int:
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16"])
     ap.add_argument("--room", default="512,384,256")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline sample (0: skip)")
+    ap.add_argument("--accuracy-steps", type=int, default=0,
+                    help="window of the FP32/FP16-vs-FP64 IR comparison (0: until the direct sound has passed "
+                         "every receiver)")
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=32, help="jobs of the batched benchmark (0: skip)")
@@ -114,6 +152,12 @@ def main() -> int:
     total = args.warmup + args.steps
     t0 = time.time()
     wl = W.config(2, room=room, nsteps=total, precision=args.precision)
+    acc_steps, last_arrival = accuracy_window(wl)
+    if args.accuracy_steps > 0:
+        acc_steps = args.accuracy_steps
+    if acc_steps > total and not args.no_fp64_check:
+        # the waveform must cover the accuracy window the run continues into
+        wl = W.config(2, room=room, nsteps=acc_steps, precision=args.precision)
     if args.tile_z or args.tile_rows:
         import dataclasses
         wl = dataclasses.replace(wl, params=dataclasses.replace(wl.params, tile_z=args.tile_z, tile_rows=args.tile_rows))
@@ -130,6 +174,9 @@ def main() -> int:
     ir_t, _, ms, kms = sv.run(args.steps, kernel_timing=True)
     wall = time.time() - t1
     tile = sv.tile_config()
+    # the accuracy window continues the same run past the timed steps
+    rest = max(0, acc_steps - total)
+    ir_rest = chunked_run(sv, rest, "%s accuracy window" % args.precision) if (rest and not args.no_fp64_check) else []
     sv.close()
     cells = wl.room_cells
     mcells = cells * args.steps / (ms * 1e-3) / 1e6
@@ -141,43 +188,55 @@ def main() -> int:
     alg_bytes_launch = info["n_upd"] * 32 * esz / 3.0
     avg_launch_s = kms * 1e-3 / launches
     achieved = alg_bytes_launch / avg_launch_s / 1e9
-    traffic = None
-    pmc = ROOT / "profiles" / ("flux_stage_%s_%s_traffic.json" % ("x".join(map(str, room)), args.precision))
-    if pmc.exists():
-        try:
-            traffic = json.loads(pmc.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+    # HBM traffic and VALU instructions per launch come from rocprofv3 counter passes of the
+    # same command (profiles/), which a profiler-free run cannot read: copied, with their source
+    tag = "%s_%s" % ("x".join(map(str, room)), args.precision)
+    pmc = profile_json("flux_stage_%s_traffic.json" % tag) or {}
+    traffic = pmc.get("hbm_bytes_per_launch")
+    valu_instr = pmc.get("valu_wave_instr_per_launch")
+    from fases_amd.solver import hbm_copy_gbs
+    copy_gbs = hbm_copy_gbs(0, 2 << 30, 5)
+    log("streaming copy (2 GiB): %.0f GB/s" % copy_gbs)
 
     from fases_amd.metrics import decay_error_db, rel_l2
-    ir = np.concatenate([ir_w, ir_t], axis=1)
-    ir_err, ir64 = None, None
+    win = max(acc_steps, total)
+    ir = np.concatenate([ir_w, ir_t] + ir_rest, axis=1)
+    ir_err, ir64, acc_note = None, None, None
     if not args.no_fp64_check and args.precision != "fp64":
         sv64 = W.make_solver(wl, "fp64")
-        ir64 = np.concatenate([sv64.run(args.warmup)[0]] + chunked_run(sv64, args.steps, "fp64 check"), axis=1)
+        ir64 = np.concatenate(chunked_run(sv64, win, "fp64 reference"), axis=1)
         sv64.close()
-        ir_err = rel_l2(ir, ir64)
-        log("IR rel L2 vs FP64 path over %d steps: %.3e (|IR|max %.3e)" % (total, ir_err, np.abs(ir64).max()))
+        ir = ir[:, :win]
+        if np.abs(ir64).max() < IR_FLOOR_PA:
+            acc_note = "FP64 IR below %.0e Pa over the window: no signal to compare" % IR_FLOOR_PA
+        else:
+            ir_err = rel_l2(ir, ir64)
+        log("IR rel L2 vs FP64 path over %d steps: %s (|IR|max %.3e)" % (win, ir_err, np.abs(ir64).max()))
+    accuracy = {"window_steps": win, "last_direct_arrival_step": last_arrival,
+                "ir64_max_pa": float(np.abs(ir64).max()) if ir64 is not None else None, "note": acc_note}
 
     fp16 = None
     if not args.no_fp16 and args.precision == "fp32":
         sv16 = W.make_solver(wl, "fp16")
         ir16w, _, _, _ = sv16.run(args.warmup)
         ir16t, _, ms16, kms16 = sv16.run(args.steps, kernel_timing=True)
+        ir16r = chunked_run(sv16, max(0, win - total), "fp16 accuracy window") if ir64 is not None else []
         log("fp16 storage run done")
         sv16.close()
-        ir16 = np.concatenate([ir16w, ir16t], axis=1)
+        ir16 = np.concatenate([ir16w, ir16t] + ir16r, axis=1)[:, :ir.shape[1]]
+        ok64 = ir64 is not None and ir_err is not None
         fp16 = {"value": round(cells * args.steps / (ms16 * 1e-3) / 1e6, 2), "unit": "Mcell-updates/s",
                 "ms_per_step": round(ms16 / args.steps, 5),
                 "flux_roofline_frac": round(info["n_upd"] * 32 * 2 / 3.0 / (kms16 * 1e-3 / launches) / 1e9
                                             / HBM_PEAK_GBS, 4),
-                "ir_rel_l2_vs_fp32": rel_l2(ir16, ir),
-                "ir_rel_l2_vs_fp64": rel_l2(ir16, ir64) if ir64 is not None else None,
-                "decay_err_db_vs_fp64": decay_error_db(ir16, ir64) if ir64 is not None else None,
-                "decay_err_db_vs_fp32": decay_error_db(ir16, ir),
+                "ir_rel_l2_vs_fp32": rel_l2(ir16, ir) if ok64 else None,
+                "ir_rel_l2_vs_fp64": rel_l2(ir16, ir64) if ok64 else None,
+                "decay_err_db_vs_fp64": decay_error_db(ir16, ir64) if ok64 else None,
+                "decay_err_db_vs_fp32": decay_error_db(ir16, ir) if ok64 else None,
                 "rescale_every": wl.params.fp16_rescale_every,
-                "note": "same room and window; decay error = max |EDC difference| down to -40 dB"}
-        log("fp16 storage: %.0f Mcell/s, IR rel L2 vs FP64 %s, vs FP32 %.3e"
+                "note": "same room and accuracy window (%d steps); decay error = max |EDC difference| down "
+                        "to -40 dB" % ir16.shape[1]}
+        log("fp16 storage: %.0f Mcell/s, IR rel L2 vs FP64 %s, vs FP32 %s"
             % (fp16["value"], fp16["ir_rel_l2_vs_fp64"], fp16["ir_rel_l2_vs_fp32"]))
 
     batch = None
@@ -252,11 +311,25 @@ def main() -> int:
         t2 = time.time()
         O.run_workload(wl, nsteps=args.cpu_steps)
         el = time.time() - t2
+        # the 64x48x40 rigid box in full (600 steps) on the CPU and on the GPU's FP64 path
+        w0 = W.config(0)
+        t3 = time.time()
+        r0 = O.run_workload(w0)["ir"]
+        el0 = time.time() - t3
+        s0 = W.make_solver(w0, "fp64")
+        g0, _, ms0, _ = s0.run(w0.nsteps)
+        s0.close()
+        c0 = {"workload": "rigid box 64x48x40 dx 0.05, Gaussian pulse, 600 steps, FP64 (full run)",
+              "cpu_value": round(w0.room_cells * w0.nsteps / el0 / 1e6, 3), "cpu_s": round(el0, 2),
+              "gpu_fp64_value": round(w0.room_cells * w0.nsteps / (ms0 * 1e-3) / 1e6, 1),
+              "gpu_vs_cpu_ir_rel_l2": rel_l2(g0, r0)}
         cpu = {"value": round(cells * args.cpu_steps / el / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
-               "kind": "port",
+               "kind": "port", "cpu_model": cpu_model(),
                "sample": "FP64 oracle (OpenMP) on the same %s room, first %d steps, %.1f s"
-                         % ("x".join(map(str, room)), args.cpu_steps, el)}
-        log("cpu baseline %.2f Mcell/s on %d threads" % (cpu["value"], threads))
+                         % ("x".join(map(str, room)), args.cpu_steps, el),
+               "rigid_box_full": c0}
+        log("cpu baseline %.2f Mcell/s on %d threads (%s); rigid box in full %.1f s, GPU FP64 vs CPU IR %.1e"
+            % (cpu["value"], threads, cpu["cpu_model"], el0, c0["gpu_vs_cpu_ir_rel_l2"]))
 
     out = {
         "metric": METRIC,
@@ -279,9 +352,23 @@ def main() -> int:
                    "tile": "%dx%dx%d" % tile},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this command, scaled by the "
+                                        "copy calibration, copied from profiles/flux_stage_%s_traffic.json" % tag)
+                                       if traffic else None,
                      "kernel": "flux stage (WENO5-Z + Godunov + SSP-RK3 + cPML)",
-                     "alg_bytes_per_launch": int(alg_bytes_launch)},
+                     "alg_bytes_per_launch": int(alg_bytes_launch),
+                     "avg_launch_us": round(avg_launch_s * 1e6, 1),
+                     "copy_kernel_gbs": round(copy_gbs, 1),
+                     "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs > 0 else None,
+                     "valu": ({"wave_instr_per_launch": valu_instr,
+                               "lane_ops_per_updated_cell": round(valu_instr * 64 / info["n_upd"], 1),
+                               "achieved_lane_ops_per_s": valu_instr * 64 / avg_launch_s,
+                               "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+                               "frac": round(valu_instr * 64 / avg_launch_s / VALU_PEAK_LANE_OPS, 4),
+                               "source": "SQ_INSTS_VALU per launch from profiles/flux_stage_%s_traffic.json, "
+                                         "launch time from this run" % tag} if valu_instr else None)},
         "ir_rel_l2_vs_fp64": ir_err,
+        "accuracy": accuracy,
         "fp16_storage": fp16,
         "batched": batch,
         "adjoint": adjoint,

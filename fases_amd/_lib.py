@@ -104,6 +104,8 @@ SIGNATURES = {
     "fases_read_mat": (C.c_int, [C.c_char_p, _flt, _P(C.c_int)]),
     "fases_estimate_t60": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_int, _flt, _flt, _flt, _flt, _flt]),
     "fases_tile_config": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
+    "fases_hbm_copy": (C.c_int, [C.c_int, C.c_int64, C.c_int, _dbl]),
+    "fases_pml_tables": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_int32), _dbl, _dbl, _dbl]),
     "fases_batch_run": (C.c_int, [_P(C.c_void_p), C.c_int, _i64, _P(_P(C.c_double)), _P(C.c_double)]),
     "fases_adjoint_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _P(C.c_double)]),
     "fases_adjoint": (C.c_int, [C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_double), _P(C.c_double)]),

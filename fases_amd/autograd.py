@@ -14,6 +14,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from . import _lib
 from ._lib import FasesError
 from .solver import Materials
 
@@ -59,6 +60,13 @@ class RoomProblem:
         self.every = checkpoint_every
         self.K, self.L = workload.materials.K, workload.materials.L
         self.cells = [workload.cell(s) for s in workload.src_ijk]
+        # fases_set_sources merges non-sine sources that share a cell (their drives add,
+        # in first-appearance order); each source's drive gradient is its merged entry's
+        first = {}
+        for c in self.cells:
+            first.setdefault(c, len(first))
+        sine = workload.src_opt == _lib.FASES_SRC_SINE
+        self.merged = np.arange(len(self.cells)) if sine else np.array([first[c] for c in self.cells], np.int64)
 
     def forward(self, theta: np.ndarray, sval: np.ndarray) -> np.ndarray:
         wl = self.wl
@@ -69,7 +77,7 @@ class RoomProblem:
 
     def backward(self, dj_dir: np.ndarray):
         gm, gs, _ = self.sv.adjoint(dj_dir)
-        return gm, gs
+        return gm, gs[self.merged]
 
 
 def free_mask(K: int, L: int, n: int) -> np.ndarray:

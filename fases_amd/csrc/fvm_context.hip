@@ -237,9 +237,54 @@ k_slices<__half>(const __half *__restrict__ f, float scale, int Nx, int Ny, int 
     out[t] = __half2float(f[c]) * scale;
 }
 
+// Streaming copy of n 16-byte words, grid-stride: the reference point for the HBM
+// roofline (every byte read once and written once).
+__global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+int fases_hbm_copy(int device, int64_t bytes, int reps, double *gbs) {
+    if (!gbs || bytes < (1 << 20) || reps < 1) return set_error(nullptr, FASES_E_ARG, "fases_hbm_copy: bad argument");
+    *gbs = 0.0;
+    const long long n = bytes / 16;
+    hipError_t e = hipSetDevice(device);
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t st = nullptr;
+    float best = 1e30f;
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&a, (size_t)n * 16);
+    if (e == hipSuccess) e = hipMalloc(&b, (size_t)n * 16);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 1, (size_t)n * 16, st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int dev_cus = 256;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, device);
+    const unsigned grid = (unsigned)std::max(1, dev_cus) * 16u;
+    for (int r = 0; r <= reps && e == hipSuccess; ++r) {     // launch 0 warms up
+        e = hipEventRecord(e0, st);
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, (const uint4 *)a, (uint4 *)b, n);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && r > 0) best = std::min(best, ms);
+    }
+    if (e == hipSuccess) *gbs = 2.0 * (double)n * 16.0 / (best * 1e-3) / 1e9;
+    const std::string msg = (e == hipSuccess) ? std::string() : std::string("fases_hbm_copy: ") + hipGetErrorString(e);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (st) (void)hipStreamDestroy(st);
+    return e == hipSuccess ? FASES_OK : set_error(nullptr, FASES_E_HIP, msg);
+}
 
 const char *fases_last_error(const fases_ctx *ctx) {
     if (ctx) return ctx->err.c_str();
@@ -385,6 +430,11 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
         std::vector<double> kc, bb, aa;
         auto axis = [&](PmlAxis &ax, int n, float lo, float hi) -> int {
             pml_axis_tables(P, n, lo, hi, band, kc, bb, aa);
+            const int a3 = (&ax == &ctx->pml.x) ? 0 : (&ax == &ctx->pml.y ? 1 : 2);
+            ctx->h_pml_band[a3] = band;
+            ctx->h_pml_kc[a3] = kc;
+            ctx->h_pml_b[a3] = bb;
+            ctx->h_pml_a[a3] = aa;
             int r;
             int *db; double *dk, *dbb, *da;
             if ((r = upload(ctx, &db, band.data(), band.size()))) return r;
@@ -434,19 +484,19 @@ int fases_set_materials(fases_ctx *ctx, int nMat, int K, int L, const double *Ri
     }
     int rc;
     ctx->nMat = nMat; ctx->K = K; ctx->L = L;
-    if ((rc = upload(ctx, &ctx->d_Rinf, Rinf, (size_t)nMat))) return rc;
-    if ((rc = upload(ctx, &ctx->d_A, A, (size_t)nMat * K))) return rc;
-    if ((rc = upload(ctx, &ctx->d_zeta, zeta, (size_t)nMat * K))) return rc;
-    if ((rc = upload(ctx, &ctx->d_B, B, (size_t)nMat * L))) return rc;
-    if ((rc = upload(ctx, &ctx->d_C, C, (size_t)nMat * L))) return rc;
-    if ((rc = upload(ctx, &ctx->d_alpha, alpha, (size_t)nMat * L))) return rc;
-    if ((rc = upload(ctx, &ctx->d_beta, beta, (size_t)nMat * L))) return rc;
-    if ((rc = upload(ctx, &ctx->d_Keff, Ke.data(), Ke.size()))) return rc;
-    if ((rc = upload(ctx, &ctx->d_Leff, Le.data(), Le.size()))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_Rinf, Rinf, (size_t)nMat))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_A, A, (size_t)nMat * K))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_zeta, zeta, (size_t)nMat * K))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_B, B, (size_t)nMat * L))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_C, C, (size_t)nMat * L))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_alpha, alpha, (size_t)nMat * L))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_beta, beta, (size_t)nMat * L))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_Keff, Ke.data(), Ke.size()))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_Leff, Le.data(), Le.size()))) return rc;
     for (int q = 0; q < 3; ++q) {
-        if ((rc = dalloc(ctx, &ctx->phi[q], (size_t)ctx->nB * K))) return rc;
-        if ((rc = dalloc(ctx, &ctx->psi1[q], (size_t)ctx->nB * L))) return rc;
-        if ((rc = dalloc(ctx, &ctx->psi2[q], (size_t)ctx->nB * L))) return rc;
+        if ((rc = dalloc_replace(ctx, &ctx->phi[q], (size_t)ctx->nB * K))) return rc;
+        if ((rc = dalloc_replace(ctx, &ctx->psi1[q], (size_t)ctx->nB * L))) return rc;
+        if ((rc = dalloc_replace(ctx, &ctx->psi2[q], (size_t)ctx->nB * L))) return rc;
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_mat = true;
@@ -463,6 +513,10 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
     if (n > 0 && src_opt == FASES_SRC_SINE && !freq)
         return set_error(ctx, FASES_E_ARG, "fases_set_sources: sine sources need freq");
     HIPCHK(hipSetDevice(ctx->prm.device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // the adjoint's source-region maps are built from the source cells: rebuild them at
+    // the next fases_adjoint_forward
+    ctx->release_adjoint();
     // merge sources that share a cell (the reference's non-atomic += would race, with the
     // same sum as the sequential result): same-cell sources add their drive amplitudes.
     std::vector<int> mc;
@@ -507,16 +561,23 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
         HIPCHK(hipMemcpyAsync(ctx->d_upd, ctx->h_upd.data(), ctx->h_upd.size(), hipMemcpyHostToDevice, ctx->stream));
         ctx->n_patch = (int)patch.size();
         ctx->h_patch = patch;
-        if (!patch.empty()) {
+        if (patch.empty()) {                       // no source any more: drop the old patch
+            if ((rc = dfree(ctx, ctx->d_patch))) return rc;
+            ctx->d_patch = nullptr;
+            for (auto &sh : ctx->shadow) {
+                if ((rc = dfree(ctx, sh))) return rc;
+                sh = nullptr;
+            }
+        } else {
             if (!ctx->d_pidx && (rc = dalloc(ctx, &ctx->d_pidx, (size_t)ctx->N))) return rc;
             std::vector<int> pidx((size_t)ctx->N, -1);
             for (size_t q = 0; q < patch.size(); ++q) pidx[(size_t)patch[q]] = (int)q;
             HIPCHK(hipMemcpyAsync(ctx->d_pidx, pidx.data(), pidx.size() * sizeof(int), hipMemcpyHostToDevice,
                                   ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));   // pidx ends with this block
-            if ((rc = upload(ctx, &ctx->d_patch, patch.data(), patch.size()))) return rc;
+            if ((rc = upload_replace(ctx, &ctx->d_patch, patch.data(), patch.size()))) return rc;
             for (int q = 0; q < 3; ++q) {
-                if ((rc = dalloc(ctx, &ctx->shadow[q], (size_t)4 * patch.size()))) return rc;
+                if ((rc = dalloc_replace(ctx, &ctx->shadow[q], (size_t)4 * patch.size()))) return rc;
                 if (ctx->esz == 4)
                     hipLaunchKernelGGL((k_patch_gather<float>), dim3((ctx->n_patch + 63) / 64), dim3(64), 0,
                                        ctx->stream, fset<float>(ctx, q), ctx->d_patch, ctx->n_patch, ctx->shadow[q],
@@ -533,11 +594,11 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
     ctx->src_opt = src_opt;
     ctx->n_samp = (src_opt == FASES_SRC_SINE) ? 0 : n_samp;
     ctx->h_sfreq = mf;
-    if ((rc = upload(ctx, &ctx->d_scell, mc.data(), mc.size()))) return rc;
-    if ((rc = upload(ctx, &ctx->d_sval, ms.data(), ms.size()))) return rc;
-    if ((rc = upload(ctx, &ctx->d_sfreq, mf.data(), mf.size()))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_scell, mc.data(), mc.size()))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_sval, ms.data(), ms.size()))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_sfreq, mf.data(), mf.size()))) return rc;
     if (src_opt != FASES_SRC_SINE && n > 0) {
-        if ((rc = upload(ctx, &ctx->d_us, u_s, (size_t)n_samp))) return rc;
+        if ((rc = upload_replace(ctx, &ctx->d_us, u_s, (size_t)n_samp))) return rc;
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return FASES_OK;
@@ -555,7 +616,7 @@ int fases_set_receivers(fases_ctx *ctx, int n, const int64_t *cell) {
     HIPCHK(hipSetDevice(ctx->prm.device));
     int rc;
     ctx->n_rec = n;
-    if ((rc = upload(ctx, &ctx->d_rcell, rc_.data(), rc_.size()))) return rc;
+    if ((rc = upload_replace(ctx, &ctx->d_rcell, rc_.data(), rc_.size()))) return rc;
     ctx->ir_cap = 0;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return FASES_OK;
@@ -637,7 +698,7 @@ int fases_get_slices(const fases_ctx *cctx, int i0, int j0, int k0, float *out) 
     const size_t per = (size_t)Ny * Nz + (size_t)Nx * Nz + (size_t)Nx * Ny;
     HIPCHK(hipSetDevice(ctx->prm.device));
     if (ctx->n_slices < 4 * per) {
-        int rc = dalloc(ctx, &ctx->d_slices, 4 * per);
+        int rc = dalloc_replace(ctx, &ctx->d_slices, 4 * per);
         if (rc) return rc;
         ctx->n_slices = 4 * per;
     }
@@ -694,6 +755,19 @@ int fases_run(fases_ctx *ctx, int64_t nsteps, double *ir, double *vel, double *e
     if (ctx->prm.precision == FASES_FP64) return run_impl<double, double>(ctx, nsteps, ir, vel, elapsed_ms, kernel_ms);
     if (ctx->prm.precision == FASES_FP16) return run_impl<__half, float>(ctx, nsteps, ir, vel, elapsed_ms, kernel_ms);
     return run_impl<float, float>(ctx, nsteps, ir, vel, elapsed_ms, kernel_ms);
+}
+
+int fases_pml_tables(const fases_ctx *ctx, int axis, int32_t *band, double *kc, double *b, double *a) {
+    if (!ctx || axis < 0 || axis > 2) return FASES_E_ARG;
+    if (!ctx->have_geo) return set_error(const_cast<fases_ctx *>(ctx), FASES_E_STATE, "fases_pml_tables: no geometry");
+    const size_t n = ctx->h_pml_band[axis].size();
+    for (size_t i = 0; i < n; ++i) {
+        if (band) band[i] = ctx->h_pml_band[axis][i];
+        if (kc) kc[i] = ctx->h_pml_kc[axis][i];
+        if (b) b[i] = ctx->h_pml_b[axis][i];
+        if (a) a[i] = ctx->h_pml_a[axis][i];
+    }
+    return FASES_OK;
 }
 
 int fases_tile_config(const fases_ctx *ctx, int *tile_x, int *tile_y, int *tile_z) {

@@ -58,8 +58,10 @@ struct fases_ctx {
            *d_alpha = nullptr, *d_beta = nullptr;
     int *d_Keff = nullptr, *d_Leff = nullptr;
     double *phi[3] = {}, *psi1[3] = {}, *psi2[3] = {};   // [0]=phi, [1]=phi_old, [2]=phi_nu
-    // PML
+    // PML (device tables in pml, host copies per axis x, y, z)
     PmlSet pml{};
+    std::vector<int> h_pml_band[3];
+    std::vector<double> h_pml_kc[3], h_pml_b[3], h_pml_a[3];
     // sources / receivers
     int n_src = 0, src_opt = FASES_SRC_RICKER;
     long long n_samp = 0;
@@ -115,6 +117,32 @@ int dalloc(fases_ctx *ctx, T **out, size_t count) {
     return FASES_OK;
 }
 
+// Frees one allocation of the context (nullptr: nothing) after the context's stream has
+// drained, and drops it from the context's list and byte count.
+int dfree(fases_ctx *ctx, void *p) {
+    if (!p) return FASES_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (size_t q = 0; q < ctx->bufs.size(); ++q)
+        if (ctx->bufs[q].ptr == p) {
+            ctx->dev_bytes -= ctx->bufs[q].bytes;
+            ctx->bufs.erase(ctx->bufs.begin() + (long)q);
+            HIPCHK(hipFree(p));
+            return FASES_OK;
+        }
+    ctx->err = "dfree: pointer not owned by the context";
+    return FASES_E_STATE;
+}
+
+// Replaces the buffer in *slot (freed, if any) by a new zeroed one of count elements, so
+// repeated set-up calls keep the context's device footprint constant.
+template <typename T>
+int dalloc_replace(fases_ctx *ctx, T **slot, size_t count) {
+    int rc = dfree(ctx, *slot);
+    *slot = nullptr;
+    if (rc) return rc;
+    return dalloc(ctx, slot, count);
+}
+
 void dfree_all(fases_ctx *ctx, void *keep_none = nullptr) {
     (void)keep_none;
     for (auto &b : ctx->bufs) if (b.ptr) (void)hipFree(b.ptr);
@@ -128,6 +156,15 @@ int upload(fases_ctx *ctx, T **dst, const T *src, size_t count) {
     if (rc) return rc;
     if (count) HIPCHK(hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
     return FASES_OK;
+}
+
+// upload() into the buffer in *slot, replacing it (see dalloc_replace).
+template <typename T>
+int upload_replace(fases_ctx *ctx, T **slot, const T *src, size_t count) {
+    int rc = dfree(ctx, *slot);
+    *slot = nullptr;
+    if (rc) return rc;
+    return upload(ctx, slot, src, count);
 }
 
 int set_error(fases_ctx *ctx, int code, const std::string &msg) {
@@ -385,15 +422,15 @@ inline int prepare_run(fases_ctx *ctx, long long nsteps) {
         for (int q = 0; q < ctx->n_src; ++q)
             for (long long t = 0; t < ns; ++t)
                 tab[(size_t)q * ns + t] = std::sin(2.0 * M_PI * ctx->h_sfreq[(size_t)q] * (double)t * (0.5 * ctx->prm.dt));
-        int rc = upload(ctx, &ctx->d_us, tab.data(), tab.size());
+        int rc = upload_replace(ctx, &ctx->d_us, tab.data(), tab.size());
         if (rc) return rc;
         ctx->n_samp = ns;
     }
     if (ctx->n_rec > 0 && nsteps > ctx->ir_cap) {
         // grow the device receiver buffers
-        int rc = dalloc(ctx, &ctx->d_ir, (size_t)ctx->n_rec * nsteps);
+        int rc = dalloc_replace(ctx, &ctx->d_ir, (size_t)ctx->n_rec * nsteps);
         if (rc) return rc;
-        rc = dalloc(ctx, &ctx->d_vel, (size_t)ctx->n_rec * nsteps * 3);
+        rc = dalloc_replace(ctx, &ctx->d_vel, (size_t)ctx->n_rec * nsteps * 3);
         if (rc) return rc;
         ctx->ir_cap = nsteps;
     }

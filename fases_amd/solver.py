@@ -169,6 +169,14 @@ def write_f32(path: str, x: np.ndarray) -> None:
     check(load().fases_write_f32(path.encode(), _ptr(x, C.c_double), x.size))
 
 
+def hbm_copy_gbs(device: int = 0, nbytes: int = 2 << 30, reps: int = 5) -> float:
+    """Read + write bandwidth [GB/s] of a streaming copy of nbytes on the device
+    (fases_hbm_copy): the measured HBM reference for the roofline figures."""
+    g = C.c_double()
+    check(load().fases_hbm_copy(device, nbytes, reps, C.byref(g)))
+    return g.value
+
+
 def cell_index(i: int, j: int, k: int, dims) -> int:
     Nx, Ny, _ = dims
     return int(i + Nx * (j + Ny * k))
@@ -254,7 +262,7 @@ class SolverParams:
     pml_kappa_max: float = 5.0
     pml_alpha_max: float = float(np.pi * 20.0)
     fp16_rescale_every: int = 100
-    use_graph: bool = False
+    use_graph: bool = False             # reserved, ignored (fases_run launches stage by stage)
     kernel_variant: int = 0             # 0 auto, 1 per-cell (reference order), 2 tiled
     tile_z: int = 0                     # tiled kernel z planes per block; 0 = autotune
     adjoint_variant: int = 0            # flux adjoint: 0 tiled, 1 per cell
@@ -392,6 +400,16 @@ class Solver:
         self._chk(self._lib.fases_adjoint(self._ctx, _ptr(dj, C.c_double), _ptr(gm, C.c_double), _ptr(gs, C.c_double),
                                           _ptr(gu, C.c_double)))
         return gm, gs[: getattr(self, "n_src", 0)], gu
+
+    def pml_tables(self, axis: int) -> dict:
+        """cPML tables of axis 0 (x), 1 (y), 2 (z) as the kernels use them: band index (-1
+        outside), kc = 1/kappa - 1, b and a of the psi update, one entry per padded cell."""
+        n = self.dims[axis]
+        band = np.zeros(n, np.int32)
+        kc, b, a = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._chk(self._lib.fases_pml_tables(self._ctx, axis, _ptr(band, C.c_int32), _ptr(kc, C.c_double),
+                                             _ptr(b, C.c_double), _ptr(a, C.c_double)))
+        return {"band": band, "kc": kc, "b": b, "a": a}
 
     def tile_config(self) -> tuple:
         """(columns, rows, z planes) per block of the tiled flux kernel (z = 0 before tuning)."""

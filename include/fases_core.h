@@ -66,7 +66,8 @@ typedef struct fases_params {
     double pml_kappa_max;   /* reference 5                                             */
     double pml_alpha_max;   /* reference 20 pi rad/s                                   */
     int    fp16_rescale_every; /* FP16: steps between per-field scale updates (100)    */
-    int    use_graph;       /* 1: replay the step loop as a captured hipGraph          */
+    int    use_graph;       /* reserved, ignored: fases_run launches stage by stage;
+                               fases_batch_run always replays captured hipGraphs       */
     int    kernel_variant;  /* 0 auto: FP64 -> per-cell kernel, FP32/FP16 -> tiled;
                                1 per-cell kernel (reference operation order, no FMA
                                contraction: bit-faithful to an FP64 CPU evaluation);
@@ -270,6 +271,13 @@ int fases_adjoint(fases_ctx *ctx, const double *dJ_dir, double *grad_mat, double
  * common/gui.py:11137-11168, each stepping main3D.cu:1280-1320 alone). */
 int fases_batch_run(fases_ctx **ctx, int n, const int64_t *nsteps, double **ir, double *elapsed_ms);
 
+/* cPML tables of one axis (0 x, 1 y, 2 z) as the kernels use them, each array of the
+ * padded axis length (Nx, Ny or Nz; any pointer may be NULL): band = tile index in
+ * [0, 2d) or -1 outside an enabled band, kc = 1/kappa - 1, b = exp(-(alpha + sigma/kappa)
+ * dt), a = sigma/(kappa (sigma + kappa alpha)) (b - 1) (functions3D.cuh cPML :681-756,
+ * cpml_adv :669-673: the reference evaluates these per cell and stage). */
+int fases_pml_tables(const fases_ctx *ctx, int axis, int32_t *band, double *kc, double *b, double *a);
+
 /* Tile of the tiled flux kernel in use: output columns x rows x z planes per block
    (*tile_z = 0 before the first fases_run when autotuning). No counterpart in the
    reference (one thread per cell there); reported so runs can state their launch shape. */
@@ -278,6 +286,12 @@ int fases_tile_config(const fases_ctx *ctx, int *tile_x, int *tile_y, int *tile_
 /* Sizes: padded cells, boundary cells, room (updated) cells, device bytes in use. */
 int fases_info(const fases_ctx *ctx, int64_t *n_cells, int64_t *n_bnd, int64_t *n_upd,
                int64_t *dev_bytes);
+
+/* HBM bandwidth reference: best of `reps` timed launches of a streaming 16-byte copy
+ * between two fresh buffers of `bytes` each on `device`; *gbs = (bytes read + bytes
+ * written) / time [GB/s]. No counterpart in the reference; the roofline reports of the
+ * time-stepping kernels are stated against it and against the published peak. */
+int fases_hbm_copy(int device, int64_t bytes, int reps, double *gbs);
 
 /* Number of HIP devices visible to this process (0 without a usable runtime or device;
    the call itself succeeds). Lets a caller pick the GPU or the host classification

@@ -110,9 +110,14 @@ struct Pml {
 };
 
 inline double cpml_corr(double D, double kappa, double psi) { return (1.0 / kappa - 1.0) * D + psi; }
+// Recursive-convolution coefficients of the psi update (functions3D.cuh:669-673).
+inline void cpml_coef(double kappa, double sigma, double alpha, double dt, double &b, double &ca) {
+    b = std::exp(-(alpha + sigma / kappa) * dt);
+    ca = (sigma > 0.0) ? (sigma / (kappa * (sigma + kappa * alpha))) * (b - 1.0) : 0.0;
+}
 inline void cpml_adv(double D, double kappa, double sigma, double alpha, double dt, double &psi) {
-    const double b = std::exp(-(alpha + sigma / kappa) * dt);
-    const double ca = (sigma > 0.0) ? (sigma / (kappa * (sigma + kappa * alpha))) * (b - 1.0) : 0.0;
+    double b, ca;
+    cpml_coef(kappa, sigma, alpha, dt, b, ca);
     psi = b * psi + ca * D;
 }
 
@@ -121,35 +126,48 @@ struct Fields {
     void init(long N) { p.assign(N, 0.0); u.assign(N, 0.0); v.assign(N, 0.0); w.assign(N, 0.0); }
 };
 
-// One profile coordinate of the cPML: depth toward the outer face, 0 inside.
-inline double depth_lo(int i, int d) { return std::fmin(1.0, std::fmax(0.0, ((double)(NG + d) - (double)i) / (double)d)); }
-inline double depth_hi(int i, int N, int d) {
-    return std::fmin(1.0, std::fmax(0.0, ((double)i - ((double)(N - 1 - NG) - (double)d)) / (double)d));
+// cPML profile of one axis at index i (functions3D.cuh:692-719): depth x in [0, 1] from
+// the interior edge (NG + d, or N-1-NG-d) to the outermost updated cell, for each enabled
+// face (key_lo at the low end, key_hi at the high end), then
+//   sigma = sigma_max x^m, kappa = 1 + (kappa_max - 1) x^m, m = 2,
+//   alpha = alpha_min + (alpha_max - alpha_min)(1 - x), alpha_min = alpha_max / 10.
+// Returns whether i lies in an enabled band (functions3D.cuh:721-723).
+bool pml_profile(const Pml &P, int N, float key_lo, float key_hi, int i, double &sig, double &kap, double &alp) {
+    const int d = P.d;
+    const double m = 2.0;
+    const double xm = std::fmin(1.0, std::fmax(0.0, ((double)(NG + d) - (double)i) / (double)d));
+    const double xp = std::fmin(1.0, std::fmax(0.0, ((double)i - ((double)(N - 1 - NG) - (double)d)) / (double)d));
+    sig = (double)key_lo * P.smax * std::pow(xm, m) + (double)key_hi * P.smax * std::pow(xp, m);
+    const double dep = (double)key_lo * xm + (double)key_hi * xp;
+    kap = 1.0 + (P.kmax - 1.0) * std::pow(dep, m);
+    alp = P.amin + (P.amax - P.amin) * (1.0 - dep);
+    return (key_lo > 0.f && i >= NG && i < NG + d) || (key_hi > 0.f && i >= N - NG - d && i <= N - 1 - NG);
+}
+
+// Face keys and profile constants (functions3D.cuh:687-719; sigma_max defaults to
+// 0.5 c (m + 1) ln(1/R0) / (d h) with R0 = 1e-7, m = 2).
+Pml make_pml(const oracle_cfg &cf) {
+    Pml P;
+    P.d = cf.pml_cells;
+    P.U = cf.pml_keys[0]; P.D = cf.pml_keys[1]; P.R = cf.pml_keys[2];
+    P.L = cf.pml_keys[3]; P.F = cf.pml_keys[4]; P.B = cf.pml_keys[5];
+    P.on = (P.U || P.D || P.R || P.L || P.F || P.B) ? 1 : 0;
+    P.smax = cf.pml_sigma_max;
+    if (!(P.smax > 0.0)) P.smax = 0.5 * cf.c * (2.0 + 1.0) * std::log(1.0 / 1e-7) / ((double)P.d * cf.h);
+    P.kmax = cf.pml_kappa_max;
+    P.amax = cf.pml_alpha_max;
+    P.amin = 0.1 * P.amax;
+    return P;
 }
 
 void cpml_cell(const G &g, const Pml &P, std::vector<double> *psi, double &p, double &u, double &v, double &w,
                double Lpx, double Lpy, double Lpz, double Lux, double Lvy, double Lwz, int i, int j, int k,
                double dt, double wssp, bool advance) {
     const int d = P.d;
-    const double m = 2.0;
-    const double xm = depth_lo(i, d), xp = depth_hi(i, g.Nx, d);
-    const double ym = depth_lo(j, d), yp = depth_hi(j, g.Ny, d);
-    const double zm = depth_lo(k, d), zp = depth_hi(k, g.Nz, d);
-    const double sig_x = (double)P.L * P.smax * std::pow(xm, m) + (double)P.R * P.smax * std::pow(xp, m);
-    const double sig_y = (double)P.B * P.smax * std::pow(ym, m) + (double)P.F * P.smax * std::pow(yp, m);
-    const double sig_z = (double)P.D * P.smax * std::pow(zm, m) + (double)P.U * P.smax * std::pow(zp, m);
-    const double dep_x = (double)P.L * xm + (double)P.R * xp;
-    const double dep_y = (double)P.B * ym + (double)P.F * yp;
-    const double dep_z = (double)P.D * zm + (double)P.U * zp;
-    const double kx = 1.0 + (P.kmax - 1.0) * std::pow(dep_x, m);
-    const double ky = 1.0 + (P.kmax - 1.0) * std::pow(dep_y, m);
-    const double kz = 1.0 + (P.kmax - 1.0) * std::pow(dep_z, m);
-    const double ax = P.amin + (P.amax - P.amin) * (1.0 - dep_x);
-    const double ay = P.amin + (P.amax - P.amin) * (1.0 - dep_y);
-    const double az = P.amin + (P.amax - P.amin) * (1.0 - dep_z);
-    const bool bx = (P.L > 0.f && i >= NG && i < NG + d) || (P.R > 0.f && i >= g.Nx - NG - d && i <= g.Nx - 1 - NG);
-    const bool by = (P.B > 0.f && j >= NG && j < NG + d) || (P.F > 0.f && j >= g.Ny - NG - d && j <= g.Ny - 1 - NG);
-    const bool bz = (P.D > 0.f && k >= NG && k < NG + d) || (P.U > 0.f && k >= g.Nz - NG - d && k <= g.Nz - 1 - NG);
+    double sig_x, kx, ax, sig_y, ky, ay, sig_z, kz, az;
+    const bool bx = pml_profile(P, g.Nx, P.L, P.R, i, sig_x, kx, ax);
+    const bool by = pml_profile(P, g.Ny, P.B, P.F, j, sig_y, ky, ay);
+    const bool bz = pml_profile(P, g.Nz, P.D, P.U, k, sig_z, kz, az);
     const double wdt = wssp * dt;
     if (bx) {
         const int ix = (i < NG + d) ? (i - NG) : (i - (g.Nx - NG - d) + d);
@@ -369,16 +387,7 @@ int oracle_run(const oracle_cfg *cf, const float *S, const uint8_t *oid, int nMa
         p1[q].assign((size_t)nB * L, 0.0);
         p2[q].assign((size_t)nB * L, 0.0);
     }
-    Pml P;
-    P.d = cf->pml_cells;
-    P.U = cf->pml_keys[0]; P.D = cf->pml_keys[1]; P.R = cf->pml_keys[2];
-    P.L = cf->pml_keys[3]; P.F = cf->pml_keys[4]; P.B = cf->pml_keys[5];
-    P.on = (P.U || P.D || P.R || P.L || P.F || P.B) ? 1 : 0;
-    P.smax = cf->pml_sigma_max;
-    if (!(P.smax > 0.0)) P.smax = 0.5 * cf->c * (2.0 + 1.0) * std::log(1.0 / 1e-7) / ((double)P.d * cf->h);
-    P.kmax = cf->pml_kappa_max;
-    P.amax = cf->pml_alpha_max;
-    P.amin = 0.1 * P.amax;
+    const Pml P = make_pml(*cf);
     std::vector<double> psi[6];
     const long d2 = 2L * P.d;
     psi[0].assign(d2 * g.Ny * g.Nz, 0.0); psi[3].assign(d2 * g.Ny * g.Nz, 0.0);
@@ -432,6 +441,20 @@ int oracle_run(const oracle_cfg *cf, const float *S, const uint8_t *oid, int nMa
 // Single reference WENO5-Z reconstruction of the 5-cell window u[0..4] centred on u[2]:
 // idx = 1 gives V^-_{i+1/2}, idx = 0 gives V^+_{i-1/2} (functions3D.cuh:863-907).
 double oracle_weno5(const double *u5, int idx) { return weno(u5 + 2, 0, 1, idx); }
+
+// cPML profile the run uses on axis `axis` (0 x: keys L/R, 1 y: B/F, 2 z: D/U) of an
+// N-cell padded axis at index i: out = {sigma, kappa, alpha, b, a} (b, a: the psi
+// update coefficients). Returns 1 in an enabled band, else 0.
+int oracle_pml_profile(const oracle_cfg *cf, int axis, int N, int i, double *out) {
+    const Pml P = make_pml(*cf);
+    const float lo = axis == 0 ? P.L : (axis == 1 ? P.B : P.D);
+    const float hi = axis == 0 ? P.R : (axis == 1 ? P.F : P.U);
+    double sig, kap, alp, b, ca;
+    const bool in = pml_profile(P, N, lo, hi, i, sig, kap, alp);
+    cpml_coef(kap, sig, alp, cf->dt, b, ca);
+    out[0] = sig; out[1] = kap; out[2] = alp; out[3] = b; out[4] = ca;
+    return in ? 1 : 0;
+}
 
 // Godunov flux pair of one face (functions3D.cuh:56-62).
 void oracle_godunov(double pM, double pP, double uM, double uP, double rho, double c, double *Fp_out,

@@ -43,6 +43,8 @@ def lib() -> C.CDLL:
         L.oracle_ade_series.restype = None
         L.oracle_ade_series.argtypes = [C.c_int, C.c_int, C.c_double] + [_dbl] * 6 + [C.c_double, _dbl, C.c_int64,
                                                                                     _dbl]
+        L.oracle_pml_profile.restype = C.c_int
+        L.oracle_pml_profile.argtypes = [_P(oracle_cfg), C.c_int, C.c_int, C.c_int, _dbl]
         _lib = L
     return _lib
 
@@ -87,6 +89,34 @@ def run(S, oid, dims, params, materials, nsteps, src_cells=(), sval=(), src_opt=
     if r != 0:
         raise RuntimeError("oracle_run failed: %d" % r)
     return {"ir": ir, "vel": vel, "fields": outs if want_fields else None}
+
+
+def _cfg(dims, params):
+    cf = oracle_cfg()
+    cf.Nx, cf.Ny, cf.Nz = dims
+    cf.h, cf.dt, cf.c, cf.rho = params.h, params.dt, params.c, params.rho
+    cf.tol = params.tol if params.tol > 0 else 3.0 * params.h
+    for i, k in enumerate(params.pml_keys):
+        cf.pml_keys[i] = float(k)
+    cf.pml_cells = params.pml_cells
+    cf.pml_sigma_max = params.pml_sigma_max
+    cf.pml_kappa_max = params.pml_kappa_max
+    cf.pml_alpha_max = params.pml_alpha_max
+    return cf
+
+
+def pml_profile(dims, params, axis: int):
+    """The oracle's cPML profile along one axis: (in_band [N] bool, prof [N, 5] = sigma,
+    kappa, alpha, b, a per padded index)."""
+    cf = _cfg(dims, params)
+    n = dims[axis]
+    inb = np.zeros(n, bool)
+    prof = np.zeros((n, 5))
+    out = np.zeros(5)
+    for i in range(n):
+        inb[i] = bool(lib().oracle_pml_profile(C.byref(cf), axis, n, i, _p(out)))
+        prof[i] = out
+    return inb, prof
 
 
 def run_workload(wl, nsteps=None, want_fields=False, want_vel=False):

@@ -107,3 +107,58 @@ def test_batch_rejects_bad_input():
         batch_run([a, a], [3, 3])
     a.close()
     b.close()
+
+
+def test_seeded_32_job_batch_equals_sequential_runs():
+    """The 32-job batched benchmark (seed 1003: rooms of 96^3 .. 256x192x160 cells, 1-3 wall
+    types, Ricker 500-4000 Hz, FP32), each job cut to a tenth of its step count (150-400
+    steps, so the jobs still drop out of the batch at different steps): the batched IRs
+    equal the jobs run one after the other through fases_run, bit for bit."""
+    js = W.batch_jobs(32)
+    steps = [wl.nsteps // 10 for wl in js]
+    announce("seeded 32-job batch, %d-%d steps per job" % (min(steps), max(steps)))
+    seq = []
+    for q, (wl, n) in enumerate(zip(js, steps)):
+        sv = W.make_solver(wl, "fp32")
+        ir, _, _, _ = sv.run(n)
+        sv.close()
+        seq.append(ir)
+        if q % 8 == 7:
+            announce("sequential: %d / 32 jobs" % (q + 1))
+    svs = [W.make_solver(wl, "fp32") for wl in js]
+    irs, ms = batch_run(svs, steps)
+    for sv in svs:
+        sv.close()
+    announce("batched: %.0f ms" % ms)
+    for q in range(len(js)):
+        assert irs[q].shape == seq[q].shape, q
+        assert np.array_equal(irs[q], seq[q]), q
+    # the jobs hear their sources within the window (the comparison is not of zeros)
+    assert sum(np.abs(ir).max() > 1e-6 for ir in irs) >= 24
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    specs = [((40, 30, 24), 160, 31, True), ((36, 44, 28), 170, 32, False), ((48, 26, 30), 150, 33, True),
+             ((30, 30, 30), 180, 34, False)]
+    js = [W.ade_room(room=room, h=0.02, nsteps=n, seed=seed, n_rec=2, K=2, L=1, pml_cells=6, pml_x_hi=pml,
+                     precision="fp32") for room, n, seed, pml in specs]
+    announce("batch of 4 uneven jobs")
+    svs = [W.make_solver(wl, "fp32") for wl in js]
+    irs, _ = batch_run(svs, [wl.nsteps for wl in js])
+    for sv in svs:
+        sv.close()
+    return js, irs
+
+
+@pytest.mark.parametrize("q", [0, 1, 2, 3])
+def test_batched_jobs_match_fp64_oracle(small_batch, q):
+    """Batched FP32 jobs of uneven size against the FP64 oracle: IR relative L2 <= 1e-5."""
+    from oracle import oracle as O
+    js, irs = small_batch
+    announce("batched job %d vs FP64 oracle" % q)
+    ref = O.run_workload(js[q])["ir"]
+    assert np.abs(ref).max() > 1e-3
+    err = np.linalg.norm(irs[q] - ref) / np.linalg.norm(ref)
+    print("job %d: IR rel L2 vs FP64 oracle %.2e" % (q, err))
+    assert err <= 1e-5
