@@ -126,17 +126,18 @@ int zero_ade(fases_ctx *ctx, const AdeState &x) {
 int build_probe_csr(fases_ctx *ctx, AdjBufs &A) {
     const int nB = ctx->nB;
     std::vector<int> base((size_t)nB), mat((size_t)nB);
-    std::vector<double> geo((size_t)nB * 8);
+    std::vector<double> geo((size_t)nB * kGeoEntries);
     HIPCHK(hipMemcpy(base.data(), ctx->d_bbase, (size_t)nB * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(mat.data(), ctx->d_bmat, (size_t)nB * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(geo.data(), ctx->d_bgeo, (size_t)nB * 64, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(geo.data(), ctx->d_bgeo, geo.size() * 8, hipMemcpyDeviceToHost));
     struct E { int tgt, b; double w; };
     std::vector<E> es;
     es.reserve((size_t)nB * 8);
     const long long nx = ctx->Nx, nxy = (long long)ctx->Nx * ctx->Ny;
     for (int b = 0; b < nB; ++b) {
         if (mat[(size_t)b] & 0x100) { es.push_back({base[(size_t)b], b, 1.0}); continue; }
-        const double sx = geo[(size_t)b * 8], sy = geo[(size_t)b * 8 + 1], sz = geo[(size_t)b * 8 + 2];
+        const double sx = geo[(size_t)kGeoSx * nB + b], sy = geo[(size_t)kGeoSy * nB + b],
+                     sz = geo[(size_t)kGeoSz * nB + b];
         for (int dz = 0; dz < 2; ++dz)
             for (int dy = 0; dy < 2; ++dy)
                 for (int dx = 0; dx < 2; ++dx) {

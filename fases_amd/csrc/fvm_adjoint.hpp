@@ -562,7 +562,7 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
 // ADE replay of one step on the boundary list: x1 = stage 0 (x0, wout(V0)),
 // x2 = stage 1 (x0, x1, wout(V1)), the same arithmetic as the forward boundary kernel.
 struct AdeState {
-    double *phi, *p1, *p2;    // [nB K], [nB L], [nB L]
+    double *phi, *p1, *p2;    // [K][nB], [L][nB], [L][nB] (pole-major, see ade_at)
 };
 
 template <typename S>
@@ -595,12 +595,13 @@ k_ade_replay(Grid g, FieldSet<S> V0, FieldSet<S> V1, Bnd<double> bd, Ade ade, do
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= bd.n) return;
     const int base = bd.base[b], mw = bd.mat[b], m = mw & 0xFF;
-    const double *q = bd.geo + 8 * (long long)b;
+    const double sx = bd.at(kGeoSx, b), sy = bd.at(kGeoSy, b), sz = bd.at(kGeoSz, b);
+    const double nx = bd.at(kGeoNx, b), ny = bd.at(kGeoNy, b), nz = bd.at(kGeoNz, b);
     const double Z0 = rho * c;
-    const double w0 = probe_wout(V0, g, base, mw, q[0], q[1], q[2], q[3], q[4], q[5], Z0, sc);
-    const double w1 = probe_wout(V1, g, base, mw, q[0], q[1], q[2], q[3], q[4], q[5], Z0, sc);
+    const double w0 = probe_wout(V0, g, base, mw, sx, sy, sz, nx, ny, nz, Z0, sc);
+    const double w1 = probe_wout(V1, g, base, mw, sx, sy, sz, nx, ny, nz, Z0, sc);
     for (int kk = 0; kk < ade.Keff[m]; ++kk) {
-        const long long s = (long long)b * ade.K + kk;
+        const long long s = ade_at(kk, b, bd.n);
         const double z = ade.zeta[m * ade.K + kk];
         const double xa = x0.phi[s];
         double xb = 0.0 * xa + 1.0 * (xa + dt * (w0 - z * xa));
@@ -611,7 +612,7 @@ k_ade_replay(Grid g, FieldSet<S> V0, FieldSet<S> V1, Bnd<double> bd, Ade ade, do
         x2.phi[s] = xc;
     }
     for (int l = 0; l < ade.Leff[m]; ++l) {
-        const long long s = (long long)b * ade.L + l;
+        const long long s = ade_at(l, b, bd.n);
         const double al = ade.alpha[m * ade.L + l], be = ade.beta[m * ade.L + l];
         const double ya = x0.p1[s], za = x0.p2[s];
         double yb = 0.0 * ya + 1.0 * (ya + dt * (w0 - (al * ya + be * za)));
@@ -646,10 +647,10 @@ k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd<double> bd, Ade ade, double rho, dou
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= bd.n) return;
     const int C = bd.cell[b], base = bd.base[b], mw = bd.mat[b], m = mw & 0xFF;
-    const double *q = bd.geo + 8 * (long long)b;
-    const double nx = q[3], ny = q[4], nz = q[5], chi = q[6];
+    const double nx = bd.at(kGeoNx, b), ny = bd.at(kGeoNy, b), nz = bd.at(kGeoNz, b), chi = bd.at(kGeoChi, b);
     const double Z0 = rho * c;
-    const double wout = probe_wout(Vin, g, base, mw, q[0], q[1], q[2], nx, ny, nz, Z0, sc);
+    const double wout = probe_wout(Vin, g, base, mw, bd.at(kGeoSx, b), bd.at(kGeoSy, b), bd.at(kGeoSz, b), nx, ny,
+                                   nz, Z0, sc);
     const int K = ade.K, L = ade.L;
     const long long nB = bd.n;
     double *Gb = G + b;                                 // entry p at Gb[p nB]
@@ -668,7 +669,7 @@ k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd<double> bd, Ade ade, double rho, dou
     Gb[0] += lwin * wout;
     const double ac = as * ic, bc = bs * ic, f = bc * dt;    // a/c, b/c, b dt/c
     for (int kk = 0; kk < ade.Keff[m]; ++kk) {
-        const long long s = (long long)b * K + kk;
+        const long long s = ade_at(kk, b, bd.n);
         const int mi = m * K + kk;
         const double z = ade.zeta[mi];
         const double x_r = xr.phi[s], x_m = xmid.phi[s];
@@ -682,7 +683,7 @@ k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd<double> bd, Ade ade, double rho, dou
         Gb[(1 + K + kk) * nB] += -f * x_r * ln;
     }
     for (int l = 0; l < ade.Leff[m]; ++l) {
-        const long long s = (long long)b * L + l;
+        const long long s = ade_at(l, b, bd.n);
         const int mi = m * L + l;
         const double al = ade.alpha[mi], be = ade.beta[mi];
         const double y1 = xr.p1[s], y2 = xr.p2[s];

@@ -403,16 +403,15 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
         const double fx = qx - (double)bi, fy = qy - (double)bj, fz = qz - (double)bk;
         const int centred = (fx == 0.0 && fy == 0.0 && fz == 0.0) ? 0x100 : 0;
         bmat.push_back(((int)oid[C] - 1) | centred);
-        bgeo.push_back(fx);
-        bgeo.push_back(fy);
-        bgeo.push_back(fz);
-        bgeo.push_back(nx);
-        bgeo.push_back(ny);
-        bgeo.push_back(nz);
-        bgeo.push_back(chi_band(s, h, tol));
-        bgeo.push_back(0.0);
+        for (double v : {fx, fy, fz, nx, ny, nz, chi_band(s, h, tol)}) bgeo.push_back(v);
     }
     ctx->nB = (int)bcell.size();
+    {   // cell-major records -> entry-major [kGeoEntries][nB]
+        std::vector<double> t(bgeo.size());
+        for (size_t b = 0; b < bcell.size(); ++b)
+            for (int f = 0; f < kGeoEntries; ++f) t[(size_t)f * bcell.size() + b] = bgeo[b * kGeoEntries + f];
+        bgeo.swap(t);
+    }
     if ((rc = upload(ctx, &ctx->d_bcell, bcell.data(), bcell.size()))) return rc;
     if ((rc = upload(ctx, &ctx->d_bbase, bbase.data(), bbase.size()))) return rc;
     if ((rc = upload(ctx, &ctx->d_bmat, bmat.data(), bmat.size()))) return rc;
@@ -448,6 +447,12 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
         if ((rc = axis(ctx->pml.x, Nx, key[3], key[2]))) return rc;   // L = x-, R = x+
         if ((rc = axis(ctx->pml.y, Ny, key[5], key[4]))) return rc;   // B = y-, F = y+
         if ((rc = axis(ctx->pml.z, Nz, key[1], key[0]))) return rc;   // D = z-, U = z+
+        const int nn[3] = {Nx, Ny, Nz};
+        const float klo[3] = {key[3], key[5], key[1]}, khi[3] = {key[2], key[4], key[0]};
+        for (int a = 0; a < 3; ++a) {
+            ctx->pml.clear0[a] = (klo[a] > 0.f) ? NG + P.pml_cells : NG;
+            ctx->pml.clear1[a] = (khi[a] > 0.f) ? nn[a] - NG - P.pml_cells : nn[a] - NG;
+        }
         const size_t d2 = 2 * (size_t)P.pml_cells;
         const size_t nx_t = ctx->pml.on ? d2 * Ny * Nz : 1, ny_t = ctx->pml.on ? (size_t)Nx * d2 * Nz : 1,
                      nz_t = ctx->pml.on ? (size_t)Nx * Ny * d2 : 1;

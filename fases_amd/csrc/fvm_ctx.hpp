@@ -57,7 +57,7 @@ struct fases_ctx {
     double *d_Rinf = nullptr, *d_A = nullptr, *d_zeta = nullptr, *d_B = nullptr, *d_C = nullptr,
            *d_alpha = nullptr, *d_beta = nullptr;
     int *d_Keff = nullptr, *d_Leff = nullptr;
-    double *phi[3] = {}, *psi1[3] = {}, *psi2[3] = {};   // [0]=phi, [1]=phi_old, [2]=phi_nu
+    double *phi[3] = {}, *psi1[3] = {}, *psi2[3] = {};   // [0]=phi, [1]=phi_old, [2]=phi_nu; [K or L][nB]
     // PML (device tables in pml, host copies per axis x, y, z)
     PmlSet pml{};
     std::vector<int> h_pml_band[3];

@@ -134,6 +134,8 @@ struct PmlAxis {
 struct PmlSet {
     int on;
     int d;
+    // per axis (x, y, z): the indices in [clear0, clear1) lie in no enabled band
+    int clear0[3], clear1[3];
     PmlAxis x, y, z;
     double *psi_px, *psi_py, *psi_pz, *psi_ux, *psi_vy, *psi_wz;
 };

@@ -250,8 +250,17 @@ template <typename G> struct Bnd {
     const int *cell;        // grid index of the boundary cell
     const int *base;        // grid index of the trilinear base corner of the probe point
     const int *mat;         // bits 0-7: material index (oid - 1); bit 8: probe on a cell centre
-    const G *geo;           // [8*n]: sx, sy, sz, nx, ny, nz, chi, pad (FP64 for FP64 fields)
+    const G *geo;           // [7][n] (entry-major): sx, sy, sz, nx, ny, nz, chi
+    // entry f of boundary cell b: consecutive cells are consecutive addresses
+    __device__ inline G at(int f, int b) const { return geo[(long long)f * n + b]; }
 };
+
+// Boundary geometry entries of Bnd::geo.
+enum BndGeo { kGeoSx = 0, kGeoSy, kGeoSz, kGeoNx, kGeoNy, kGeoNz, kGeoChi, kGeoEntries };
+
+// ADE state k of boundary cell b in an [K][nB] state buffer (pole-major, so the lanes of a
+// wave, consecutive boundary cells, touch consecutive addresses).
+__device__ inline long long ade_at(int k, int b, int nB) { return (long long)k * nB + b; }
 
 struct Ade {
     int K, L;
@@ -304,8 +313,7 @@ __device__ inline void boundary_cell(int b, const Grid &g, const FieldSet<S> &in
     typedef Conv<S, R> CV;
     const int C = bd.cell[b], base = bd.base[b], mw = bd.mat[b];
     const int m = mw & 0xFF;
-    const G *q = bd.geo + 8 * (long long)b;
-    const double sx = q[0], sy = q[1], sz = q[2], nx = q[3], ny = q[4], nz = q[5], chi = q[6];
+    const double nx = bd.at(kGeoNx, b), ny = bd.at(kGeoNy, b), nz = bd.at(kGeoNz, b), chi = bd.at(kGeoChi, b);
     double pD, uD, vD, wD;
     if (mw & 0x100) {
         // probe exactly on a cell centre (all trilinear fractions 0): the interpolant is
@@ -315,6 +323,7 @@ __device__ inline void boundary_cell(int b, const Grid &g, const FieldSet<S> &in
         vD = (double)CV::ld(in.v[base], (R)sc.s[2]);
         wD = (double)CV::ld(in.w[base], (R)sc.s[3]);
     } else {
+        const double sx = bd.at(kGeoSx, b), sy = bd.at(kGeoSy, b), sz = bd.at(kGeoSz, b);
         pD = tri_probe(in.p, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[0]);
         uD = tri_probe(in.u, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[1]);
         vD = tri_probe(in.v, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[2]);
@@ -324,28 +333,30 @@ __device__ inline void boundary_cell(int b, const Grid &g, const FieldSet<S> &in
     const double utx = uD - un * nx, uty = vD - un * ny, utz = wD - un * nz;
     const double Z0 = rho * c;
     const double wout = pD + Z0 * un;
-    // ADE (vector-fitted admittance): real poles phi, complex pairs (psi1, psi2)
+    // ADE (vector-fitted admittance): real poles phi, complex pairs (psi1, psi2). A stage
+    // with a = 0 (the first) does not read the mid buffer: its term is a zero either way.
     double sumA = 0.0, sumBC = 0.0;
     const int Ke = ade.Keff[m], Le = ade.Leff[m];
+    const bool mid = as != 0.0;
     for (int k = 0; k < Ke; ++k) {
         const int mi = m * ade.K + k;
-        const long long s = (long long)b * ade.K + k;
+        const long long s = ade_at(k, b, bd.n);
         const double xr = phi_r[s];
         const double Lphi = wout - ade.zeta[mi] * xr;
-        double xl = as * phi_m[s] + bs * (xr + dt * Lphi);
+        double xl = as * (mid ? phi_m[s] : 0.0) + bs * (xr + dt * Lphi);
         xl /= cs;
         phi_l[s] = xl;
         sumA += ade.A[mi] * xl;
     }
     for (int l = 0; l < Le; ++l) {
         const int mi = m * ade.L + l;
-        const long long s = (long long)b * ade.L + l;
+        const long long s = ade_at(l, b, bd.n);
         const double y1 = p1_r[s], y2 = p2_r[s];
         const double L1 = wout - (ade.alpha[mi] * y1 + ade.beta[mi] * y2);
         const double L2 = ade.beta[mi] * y1 - ade.alpha[mi] * y2;
-        double z1 = as * p1_m[s] + bs * (y1 + dt * L1);
+        double z1 = as * (mid ? p1_m[s] : 0.0) + bs * (y1 + dt * L1);
         z1 /= cs;
-        double z2 = as * p2_m[s] + bs * (y2 + dt * L2);
+        double z2 = as * (mid ? p2_m[s] : 0.0) + bs * (y2 + dt * L2);
         z2 /= cs;
         p1_l[s] = z1;
         p2_l[s] = z2;

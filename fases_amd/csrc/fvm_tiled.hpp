@@ -34,20 +34,30 @@ template <> __device__ inline double fast_rcp<double>(double x) {
     return r;
 }
 
-// Both WENO5-Z states of the cell whose 5-cell window is (a,b,c,d,e), centre c:
+// Smoothness term of a 3-cell stencil with second difference s: 13/3 s^2 + 4 eps. Each
+// stencil serves three cells (as beta_0, beta_1, beta_2), so a pipeline along an axis
+// computes it once per cell and carries it to the next two.
+template <typename R>
+__device__ inline R weno_q(R s) {
+    const R k = R(13) / R(3);
+    return fma(k * s, s, R(4e-10));
+}
+
+// Both WENO5-Z states of a cell from the first differences d1..d4 of its 5-cell window
+// (a,b,c,d,e): d1 = b - a, ..., d4 = e - d, the smoothness terms q0 = weno_q(d4 - d3),
+// q1 = weno_q(d3 - d2), q2 = weno_q(d2 - d1) and its centre c:
 //   A = V^-_{i+1/2} (linear weights 3/10, 3/5, 1/10 on the stencils ending at c+2, c+1, c)
 //   B = V^+_{i-1/2} (mirrored weights 1/10, 3/5, 3/10)
+// Every caller goes through this function, so a face state is the same value whether
+// its differences were computed here or carried along an axis.
 template <typename R>
-__device__ inline void weno_pair(R a, R b, R c, R d, R e, R &A, R &B) {
-    const R d1 = b - a, d2 = c - b, d3 = d - c, d4 = e - d;
-    const R s0 = d4 - d3, s1 = d3 - d2, s2 = d2 - d1;              // second differences
+__device__ inline void weno_core(R d1, R d2, R d3, R d4, R q0, R q1, R q2, R c, R &A, R &B) {
     const R t0 = fma(R(-3), d3, d4), t1 = d2 + d3, t2 = fma(R(3), d2, -d1);
-    // 4 x smoothness indicators: 13/3 s^2 + t^2 (the weights only see ratios of
+    // 4 x smoothness indicators: 13/3 s^2 + t^2 + 4 eps (the weights only see ratios of
     // beta + eps and |beta_0 - beta_2|, so beta, tau and eps all carry the factor 4)
-    const R k = R(13) / R(3);
-    const R e0 = fma(k * s0, s0, fma(t0, t0, R(4e-10)));
-    const R e1 = fma(k * s1, s1, fma(t1, t1, R(4e-10)));
-    const R e2 = fma(k * s2, s2, fma(t2, t2, R(4e-10)));
+    const R e0 = fma(t0, t0, q0);
+    const R e1 = fma(t1, t1, q1);
+    const R e2 = fma(t2, t2, q2);
     const R tau = fabs(e0 - e2);
     // unnormalised WENO-Z weights d_r (1 + tau/e_r) times e0 e1 e2:
     // g_r = (e_r + tau) * prod_{s != r} e_s
@@ -70,6 +80,36 @@ __device__ inline void weno_pair(R a, R b, R c, R d, R e, R &A, R &B) {
     A = fma(nl, fast_rcp(dl), c);
     B = fma(nr, fast_rcp(dr), c);
 }
+
+// Both WENO5-Z states of the cell whose 5-cell window is (a,b,c,d,e), centre c.
+template <typename R>
+__device__ inline void weno_pair(R a, R b, R c, R d, R e, R &A, R &B) {
+    const R d1 = b - a, d2 = c - b, d3 = d - c, d4 = e - d;
+    weno_core(d1, d2, d3, d4, weno_q(d4 - d3), weno_q(d3 - d2), weno_q(d2 - d1), c, A, B);
+}
+
+// One variable's window along the z march, carried from plane to plane: the centres
+// u[k], u[k+1], u[k+2], the differences d1 = u[k]-u[k-1], d2, d3 = u[k+2]-u[k+1] and the
+// smoothness terms of the stencils centred on k+1 (q1) and k (q2). With u[k+3] the pair
+// of cell k+1 needs one new difference and one new smoothness term.
+template <typename R> struct ZCarry {
+    R c0, c1, c2, d1, d2, d3, q1, q2;
+    __device__ inline void init(const R *u) {      // u[0..5] = planes k-3 .. k+2
+        c0 = u[3]; c1 = u[4]; c2 = u[5];
+        d1 = u[3] - u[2]; d2 = u[4] - u[3]; d3 = u[5] - u[4];
+        q1 = weno_q(d3 - d2);
+        q2 = weno_q(d2 - d1);
+    }
+    // states of cell k+1 from u[k+3]; then the window moves one plane on
+    __device__ inline void step(R un, R &A, R &B) {
+        const R d4 = un - c2;
+        const R q0 = weno_q(d4 - d3);
+        weno_core(d1, d2, d3, d4, q0, q1, q2, c1, A, B);
+        c0 = c1; c1 = c2; c2 = un;
+        d1 = d2; d2 = d3; d3 = d4;
+        q2 = q1; q1 = q0;
+    }
+};
 
 namespace tile {
 constexpr int TXO = 62;         // output columns per block
@@ -232,8 +272,10 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         hdst[q] = (h_kind == 0) ? &L.p[h_r][h_c] : (h_kind == 1 ? &L.u[h_r][h_c] : &L.v[h_r][h_c]);
     }
 
-    // z pipeline: window p/w at k-1..k+3, carried A-state of cell k and flux through k-1/2
-    R pz[5], wz[5];
+    // z pipeline: carried windows of p and w (centres k..k+2, differences, smoothness
+    // terms), the newest plane k+3 (pn, wn), the A-state of cell k and the flux through k-1/2
+    ZCarry<R> zp, zw;
+    R pn = R(0), wn = R(0);
     R pA, wA, FzP, FzW;
     {
         R p7[6], w7[6];
@@ -249,9 +291,8 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         weno_pair(p7[1], p7[2], p7[3], p7[4], p7[5], pA, pB0);     // cell k0
         weno_pair(w7[1], w7[2], w7[3], w7[4], w7[5], wA, wB0);
         godunov_f(pAm, pB0, wAm, wB0, K, FzP, FzW);            // face k0-1/2
-#pragma unroll
-        for (int q = 0; q < 4; ++q) { pz[q] = p7[q + 2]; wz[q] = w7[q + 2]; }
-        pz[4] = wz[4] = R(0);                  // plane k0+3: from pend_p / pend_w below
+        zp.init(p7);
+        zw.init(w7);
     }
     // values in flight, in storage precision: own u, v, update flag, base set (stages 1,
     // 2), halo items, and p, w of plane k+3 for the z window. Each plane converts what
@@ -278,28 +319,29 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         const long long p3 = nxy * (long long)(k0 + 3);
         pend_p = LDR(ip, p3);
         pend_w = LDR(iw, p3);
-        if constexpr (!kRaw) {                 // this order fills slots at the plane's end
-            pz[4] = CV::ld(pend_p, sc.s[0]);
-            wz[4] = CV::ld(pend_w, sc.s[3]);
+        if constexpr (!kRaw) {                 // this order converts at the plane's end
+            pn = CV::ld(pend_p, sc.s[0]);
+            wn = CV::ld(pend_w, sc.s[3]);
         }
     }
+    // cPML: whether the tile's columns or rows reach a band (uniform per block); the
+    // plane's z test is added per plane, so tiles clear of every band skip pml_apply
+    const bool pml_xy = pml.on && (x0 < pml.clear0[0] || x0 + TXO > pml.clear1[0] || j0 < pml.clear0[1] ||
+                                   j0 + NW > pml.clear1[1]);
 
-    // One plane k. PH = (k - k0) mod 5 is a compile-time constant: logical window slot q
-    // (plane k - 1 + q) lives in register pz[(q + PH) % 5].
-    auto plane = [&](int k, auto phase) __device__ __attribute__((always_inline)) {
-        constexpr int PH = decltype(phase)::value;
+    // One plane k.
+    auto plane = [&](int k) __device__ __attribute__((always_inline)) {
         const long long pl = nxy * (long long)k;
         // ---- the values the previous plane loaded, into compute precision. With reduced
-        // storage the z window takes plane k+3 into its logical slot 4 (the slot plane
-        // k-2 held) here, and the next plane's loads go out at once, a whole plane ahead
-        // of their conversion; with FP32 / FP64 storage (no conversion) the loads go out
-        // after the first barrier and the slot is filled at the end of the plane
-        // (measured faster each way).
+        // storage the z window's newest plane k+3 is converted here, and the next plane's
+        // loads go out at once, a whole plane ahead of their conversion; with FP32 / FP64
+        // storage (no conversion) the loads go out after the first barrier and the value is
+        // taken at the end of the plane (measured faster each way).
         if constexpr (kRaw) {
-            pz[(PH + 4) % 5] = CV::ld(pend_p, sc.s[0]);
-            wz[(PH + 4) % 5] = CV::ld(pend_w, sc.s[3]);
+            pn = CV::ld(pend_p, sc.s[0]);
+            wn = CV::ld(pend_w, sc.s[3]);
         }
-        const R pc = pz[(1 + PH) % 5], wc = wz[(1 + PH) % 5];
+        const R pc = zp.c0, wc = zw.c0;
         const R uc = CV::ld(nu_s, sc.s[1]), vc = CV::ld(nv_s, sc.s[2]);
         const S b0 = nb0, b1 = nb1, b2 = nb2, b3 = nb3;
         const unsigned flag = nflag;
@@ -376,12 +418,10 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         godunov_f(L.ys[0][wy + 1][lx], L.ys[1][wy + 2][lx], L.ys[2][wy + 1][lx], L.ys[3][wy + 2][lx], K, FyP_p,
                   FyP_v);
         godunov_f(L.ys[0][wy][lx], L.ys[1][wy + 1][lx], L.ys[2][wy][lx], L.ys[3][wy + 1][lx], K, FyM_p, FyM_v);
-        // ---- z face k+1/2 from the new cell k+1
+        // ---- z face k+1/2 from the new cell k+1 (the carried windows move on a plane)
         R pAn, wAn, pBn, wBn, FzP_p, FzP_w;
-        weno_pair(pz[(0 + PH) % 5], pz[(1 + PH) % 5], pz[(2 + PH) % 5], pz[(3 + PH) % 5], pz[(4 + PH) % 5], pAn,
-                  pBn);
-        weno_pair(wz[(0 + PH) % 5], wz[(1 + PH) % 5], wz[(2 + PH) % 5], wz[(3 + PH) % 5], wz[(4 + PH) % 5], wAn,
-                  wBn);
+        zp.step(pn, pAn, pBn);
+        zw.step(wn, wAn, wBn);
         godunov_f(pA, pBn, wA, wBn, K, FzP_p, FzP_w);
         if (out_lane && flag == 1) {
             const R sp = -FxP_p + FxM_p - FyP_p + FyM_p - FzP_p + FzP;
@@ -410,7 +450,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
                     wgt = R(2) / R(3);
                 }
             }
-            if (pml.on)
+            if (pml_xy || (pml.on && (k < pml.clear0[2] || k >= pml.clear1[2])))
                 pml_apply<R, STAGE == 2>(pml, i, j, k, Nx, Ny, po, uo, vo, wo, (-FxP_p + FxM_p) * K.ih,
                                          (-FyP_p + FyM_p) * K.ih, (-FzP_p + FzP) * K.ih, Du * K.ih, Dv * K.ih,
                                          Dw * K.ih, wgt * dt);
@@ -419,26 +459,28 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
             ST(ow, pl, wo, sc.is[3]);
             ST(op, pl, po, sc.is[0]);
         }
-        // advance the z pipeline: the window is a ring indexed at compile time, so the
-        // advance moves no registers (plane k+4 takes slot PH, which plane k-1 held)
+        // advance the z pipeline
         pA = pAn; wA = wAn; FzP = FzP_p; FzW = FzP_w;
         if constexpr (!kRaw) {
-            pz[PH] = CV::ld(pend_p, sc.s[0]);
-            wz[PH] = CV::ld(pend_w, sc.s[3]);
+            pn = CV::ld(pend_p, sc.s[0]);
+            wn = CV::ld(pend_w, sc.s[3]);
         }
     };
-    // five planes per trip so the ring returns to its start; kend is uniform, so every
-    // wave leaves at the same plane and the barriers stay matched
-    for (int k = k0; k < kend; k += 5) {
-        plane(k, std::integral_constant<int, 0>{});
+    // six planes per trip (the carried windows rotate with periods 3 and 2, so the
+    // compiler renames instead of moving); kend is uniform, so every wave leaves at the
+    // same plane and the barriers stay matched
+    for (int k = k0; k < kend; k += 6) {
+        plane(k);
         if (k + 1 >= kend) break;
-        plane(k + 1, std::integral_constant<int, 1>{});
+        plane(k + 1);
         if (k + 2 >= kend) break;
-        plane(k + 2, std::integral_constant<int, 2>{});
+        plane(k + 2);
         if (k + 3 >= kend) break;
-        plane(k + 3, std::integral_constant<int, 3>{});
+        plane(k + 3);
         if (k + 4 >= kend) break;
-        plane(k + 4, std::integral_constant<int, 4>{});
+        plane(k + 4);
+        if (k + 5 >= kend) break;
+        plane(k + 5);
     }
 }
 
