@@ -99,6 +99,7 @@ SIGNATURES = {
     "fases_info": (C.c_int, [C.c_void_p, _i64, _i64, _i64, _i64]),
     "fases_device_count": (C.c_int, [_P(C.c_int)]),
     "fases_get_slices": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _flt]),
+    "fases_get_slices_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _flt]),
     "fases_scene_group_areas": (C.c_int, [C.c_void_p, _flt, _flt]),
     "fases_air_attenuation": (C.c_int, [C.c_float, C.c_float, C.c_float, _flt, C.c_int, _flt]),
     "fases_read_mat": (C.c_int, [C.c_char_p, _flt, _P(C.c_int)]),

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -693,23 +694,28 @@ int fases_reset(fases_ctx *ctx) {
     return fases_set_fields(ctx, nullptr, nullptr, nullptr, nullptr);
 }
 
-int fases_get_slices(const fases_ctx *cctx, int i0, int j0, int k0, float *out) {
-    fases_ctx *ctx = const_cast<fases_ctx *>(cctx);
+}  // extern "C"
+
+namespace {
+
+// Checks of both slice entry points; *per = floats of the three planes of one field.
+int slices_args(fases_ctx *ctx, const char *who, int i0, int j0, int k0, const float *out, size_t *per) {
     if (!ctx || !out) return FASES_E_ARG;
-    if (!ctx->have_geo) return set_error(ctx, FASES_E_STATE, "fases_get_slices: no geometry");
+    if (!ctx->have_geo) return set_error(ctx, FASES_E_STATE, std::string(who) + ": no geometry");
     const int Nx = ctx->Nx, Ny = ctx->Ny, Nz = ctx->Nz;
     if (i0 < 0 || i0 >= Nx || j0 < 0 || j0 >= Ny || k0 < 0 || k0 >= Nz)
-        return set_error(ctx, FASES_E_RANGE, "fases_get_slices: slice plane outside the grid");
-    const size_t per = (size_t)Ny * Nz + (size_t)Nx * Nz + (size_t)Nx * Ny;
-    HIPCHK(hipSetDevice(ctx->prm.device));
-    if (ctx->n_slices < 4 * per) {
-        int rc = dalloc_replace(ctx, &ctx->d_slices, 4 * per);
-        if (rc) return rc;
-        ctx->n_slices = 4 * per;
-    }
+        return set_error(ctx, FASES_E_RANGE, std::string(who) + ": slice plane outside the grid");
+    *per = (size_t)Ny * Nz + (size_t)Nx * Nz + (size_t)Nx * Ny;
+    return FASES_OK;
+}
+
+// Gathers the planes of p, u, v, w into the device buffer dst [4 * per] and copies them
+// to the host array out (one copy), on the context's stream; returns once out is filled.
+int slices_to_host(fases_ctx *ctx, int i0, int j0, int k0, size_t per, float *dst_dev, float *out) {
+    const int Nx = ctx->Nx, Ny = ctx->Ny, Nz = ctx->Nz;
     const unsigned nb = (unsigned)((per + 255) / 256);
     for (int f = 0; f < 4; ++f) {
-        float *dst = ctx->d_slices + (size_t)f * per;
+        float *dst = dst_dev + (size_t)f * per;
         if (ctx->esz == 8)
             hipLaunchKernelGGL(k_slices<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double *)ctx->U[0][f],
                                1.0f, Nx, Ny, Nz, i0, j0, k0, dst);
@@ -721,9 +727,74 @@ int fases_get_slices(const fases_ctx *cctx, int i0, int j0, int k0, float *out) 
                                (float)std::ldexp(1.0, ctx->fexp[f]), Nx, Ny, Nz, i0, j0, k0, dst);
         HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipMemcpyAsync(out, ctx->d_slices, 4 * per * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, dst_dev, 4 * per * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return FASES_OK;
+}
+
+}  // namespace
+
+// Staging buffer of fases_get_slices_noalloc: a file-scope device array, one copy per GPU
+// the library runs on, held until the process ends. In host code its name is not a device
+// address: host code uses only the per-device pointer from hipGetSymbolAddress.
+constexpr size_t kSliceScratchFloats = (size_t)1 << 21;      // 8 MiB: 2^21 floats
+static __device__ __align__(256) float scratch[kSliceScratchFloats];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the context's device, resolved once per device with that
+// device current.
+int slice_scratch(fases_ctx *ctx, float **ptr) {
+    const int dev = ctx->prm.device;
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return set_error(ctx, FASES_E_ARG, "fases_get_slices_noalloc: device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        HIPCHK(hipSetDevice(dev));
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    *ptr = g_scratch_ptr[dev];
+    return FASES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fases_get_slices(const fases_ctx *cctx, int i0, int j0, int k0, float *out) {
+    fases_ctx *ctx = const_cast<fases_ctx *>(cctx);
+    size_t per = 0;
+    int rc = slices_args(ctx, "fases_get_slices", i0, j0, k0, out, &per);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->prm.device));
+    if (ctx->n_slices < 4 * per) {
+        rc = dalloc_replace(ctx, &ctx->d_slices, 4 * per);
+        if (rc) return rc;
+        ctx->n_slices = 4 * per;
+    }
+    return slices_to_host(ctx, i0, j0, k0, per, ctx->d_slices, out);
+}
+
+int fases_get_slices_noalloc(const fases_ctx *cctx, int i0, int j0, int k0, float *out) {
+    fases_ctx *ctx = const_cast<fases_ctx *>(cctx);
+    size_t per = 0;
+    int rc = slices_args(ctx, "fases_get_slices_noalloc", i0, j0, k0, out, &per);
+    if (rc) return rc;
+    if (4 * per > kSliceScratchFloats)
+        return set_error(ctx, FASES_E_ARG,
+                         "fases_get_slices_noalloc: the planes need " + std::to_string(4 * per) +
+                             " floats, above its static staging buffer of " + std::to_string(kSliceScratchFloats) +
+                             " (use fases_get_slices)");
+    HIPCHK(hipSetDevice(ctx->prm.device));
+    float *buf = nullptr;
+    if ((rc = slice_scratch(ctx, &buf))) return rc;
+    return slices_to_host(ctx, i0, j0, k0, per, buf, out);
 }
 
 int fases_get_fields(const fases_ctx *cctx, double *p, double *u, double *v, double *w) {

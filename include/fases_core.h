@@ -232,6 +232,14 @@ int fases_get_fields(const fases_ctx *ctx, double *p, double *u, double *v, doub
  * Gathered on the device, one copy to the host. A plane outside the grid is FASES_E_RANGE. */
 int fases_get_slices(const fases_ctx *ctx, int i0, int j0, int k0, float *out);
 
+/* fases_get_slices without any device allocation: the planes are gathered into a static
+ * device staging array of 2^21 floats (8 MiB) that each GPU the library runs on holds until
+ * the process ends; grids whose 4 * (Ny*Nz + Nx*Nz + Nx*Ny) exceeds that are FASES_E_ARG.
+ * No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under hipGraph capture, serving loops, harnesses that forbid hipMalloc). Calls on one GPU
+ * share the array: calls that overlap in time on the same GPU race. */
+int fases_get_slices_noalloc(const fases_ctx *ctx, int i0, int j0, int k0, float *out);
+
 /* Zero fields, ADE and PML states, step counter 0. */
 int fases_reset(fases_ctx *ctx);
 
