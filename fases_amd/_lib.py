@@ -50,6 +50,19 @@ class fases_params(C.Structure):
     ]
 
 
+class fases_ga_params(C.Structure):
+    _fields_ = [
+        ("device", C.c_int),
+        ("ism_order", C.c_int),
+        ("n_rays", C.c_int),
+        ("rt_max_order", C.c_int),
+        ("recv_radius", C.c_float),
+        ("c", C.c_float),
+        ("fs", C.c_float),
+        ("Nt", C.c_int),
+    ]
+
+
 class FasesError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"fases error {code}: {msg}")
@@ -106,6 +119,17 @@ SIGNATURES = {
     "fases_estimate_t60": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_int, _flt, _flt, _flt, _flt, _flt]),
     "fases_tile_config": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
     "fases_hbm_copy": (C.c_int, [C.c_int, C.c_int64, C.c_int, _dbl]),
+    "fases_ga_params_default": (C.c_int, [_P(fases_ga_params)]),
+    "fases_ga_run": (C.c_int, [C.c_void_p, _P(fases_ga_params), C.c_int, _flt, _flt, C.c_int, _flt, _flt, _flt, _flt,
+                               _P(C.c_void_p)]),
+    "fases_ga_free": (None, [C.c_void_p]),
+    "fases_ga_info": (C.c_int, [C.c_void_p, _P(C.c_int), _i64, _i64, _dbl, _dbl, _dbl]),
+    "fases_ga_direct": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _flt]),
+    "fases_ga_images": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _P(C.c_int), _P(C.c_int), _flt, _flt, _flt,
+                                  _P(C.c_int), _P(C.c_int), _flt, _flt]),
+    "fases_ga_energy": (C.c_int, [C.c_void_p, _flt, _flt]),
+    "fases_ga_band_irs": (C.c_int, [C.c_void_p, C.c_int, _flt, _flt]),
+    "fases_ga_arrivals": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_int), _flt]),
     "fases_pml_tables": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_int32), _dbl, _dbl, _dbl]),
     "fases_batch_run": (C.c_int, [_P(C.c_void_p), C.c_int, _i64, _P(_P(C.c_double)), _P(C.c_double)]),
     "fases_adjoint_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _P(C.c_double)]),

@@ -31,10 +31,11 @@ HIP_FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics", "-Wno-unused-result", "-Wno-cuda-compat"]
 
-HOST_SOURCES = ["host_formats.cpp", "host_geometry.cpp"]
-HIP_SOURCES = ["fvm_context.hip", "fvm_batch.hip", "fvm_adjoint.hip", "fvm_geometry.hip"]
-# The geometry kernels reproduce the host classification bit for bit: no FMA contraction.
-HIP_EXTRA = {"fvm_geometry.hip": ["-ffp-contract=off"]}
+HOST_SOURCES = ["host_formats.cpp", "host_geometry.cpp", "host_ga.cpp"]
+HIP_SOURCES = ["fvm_context.hip", "fvm_batch.hip", "fvm_adjoint.hip", "fvm_geometry.hip", "fvm_ga.hip"]
+# The geometry and geometrical-acoustics kernels reproduce the host's float decisions bit
+# for bit: no FMA contraction.
+HIP_EXTRA = {"fvm_geometry.hip": ["-ffp-contract=off"], "fvm_ga.hip": ["-ffp-contract=off"]}
 
 
 def _run(cmd: list[str]) -> None:

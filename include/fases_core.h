@@ -301,6 +301,72 @@ int fases_info(const fases_ctx *ctx, int64_t *n_cells, int64_t *n_bnd, int64_t *
  * time-stepping kernels are stated against it and against the published peak. */
 int fases_hbm_copy(int device, int64_t bytes, int reps, double *gbs);
 
+/* ------------------------------------------------------------------------- */
+/* Geometrical acoustics: image sources + ray tracer (host C++ and HIP)       */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fases_ga fases_ga;
+
+typedef struct fases_ga_params {
+    int   device;        /* HIP device of the path-validity and ray-tracing kernels; < 0:
+                            image sources on the host, no ray tracer                    */
+    int   ism_order;     /* image-source order (config line 13, main3D.cu:1484; <= 32),
+                            reduced when the image tree would not fit in half the free
+                            host memory (ism3D.h:539-559)                              */
+    int   n_rays;        /* rays per source (config line 15, main3D.cu:1540); 0: none   */
+    int   rt_max_order;  /* reflections per ray at most (main3D.cu:1541: 100000)        */
+    float recv_radius;   /* detection-sphere radius [m] (main3D.cu:1543: 0.12)          */
+    float c;             /* speed of sound [m/s]                                        */
+    float fs;            /* sample rate of the echograms [Hz] (main3D.cu:1508: 44100)   */
+    int   Nt;            /* samples (main3D.cu:1510: t_max fs)                          */
+} fases_ga_params;
+
+/* Fills *p with the reference's defaults (order 3, 10000 rays, radius 0.12 m, 44.1 kHz). */
+int fases_ga_params_default(fases_ga_params *p);
+
+/* Image sources and ray tracing of one scene for n_src sources and n_rec receivers given
+ * in the scene's raw (OBJ) coordinates, placed like the mesh (ism3D.h ism_solve :665,
+ * raytrace3D.h rt_build :229 + rt_trace_late :335, driven as main3D.cu:1479-1585 drives
+ * them). src_pressure [n_src] (nullable: 1 Pa): pressure at 1 m; alpha [n_groups*8]
+ * (nullable: rigid): octave-band absorption per mesh group (the .mat values); sctr
+ * [n_groups] (nullable: none): scattering per group (the .sctr values); m_air [8]
+ * (nullable: none): air attenuation [Np/m]. Paths of order <= the image-source order
+ * belong to the image sources; the ray tracer keeps every higher specular order and all
+ * scattered energy. */
+int fases_ga_run(const fases_scene *scene, const fases_ga_params *p, int n_src, const float *src_xyz,
+                 const float *src_pressure, int n_rec, const float *rec_xyz, const float *alpha,
+                 const float *sctr, const float *m_air, fases_ga **out);
+void fases_ga_free(fases_ga *g);
+
+/* Image-source order used, image-tree nodes, valid image paths, and the host times of the
+ * image sources, of the ray tracer and of its kernel [ms] (any pointer may be NULL). */
+int fases_ga_info(const fases_ga *g, int *order_used, int64_t *n_nodes, int64_t *n_paths, double *ism_ms,
+                  double *rt_ms, double *rt_kernel_ms);
+
+/* Direct source-receiver distance [m], 0 when occluded (ISMResult::direct). */
+int fases_ga_direct(const fases_ga *g, int s, int r, float *dist);
+
+/* Image paths of (s, r), by increasing length (ISMResult::images): count and total
+ * bounces (call with NULL arrays first), then per path the unfolded distance, the unit
+ * arrival direction (receiver to image, placed frame) [3], the curvature gain and the
+ * order; per bounce in source-to-receiver order the hit group, incidence cosine and angle. */
+int fases_ga_images(const fases_ga *g, int s, int r, int *n_images, int *n_bounces, float *distance, float *doa,
+                    float *amp, int *order, int *group, float *cos_inc, float *angle);
+
+/* Ray-traced energy echograms e_spec, e_diff [8 * n_rec * Nt], index (b n_rec + r) Nt + t. */
+int fases_ga_energy(const fases_ga *g, float *e_spec, float *e_diff);
+
+/* Octave band `band` (0 = 63 Hz .. 7 = 8 kHz) of the geometrical IR of every receiver,
+ * p_hf [n_rec * Nt] (the HIR file: image sources as windowed-sinc fractional delays plus
+ * the ray-traced energy as pressure with a random sign) and p_rt [n_rec * Nt] (nullable;
+ * the HRT file: the ray-traced part alone) (main3D.cu:1600-1757). */
+int fases_ga_band_irs(const fases_ga *g, int band, float *p_hf, float *p_rt);
+
+/* Discrete arrivals at receiver r (the ARR file, main3D.cu:1759-1801): *n arrivals (buf
+ * NULL to count), each 12 floats: delay [s], unit direction receiver -> (image) source
+ * [3], pressure gain per octave band [8]. */
+int fases_ga_arrivals(const fases_ga *g, int r, int *n, float *buf);
+
 /* Number of HIP devices visible to this process (0 without a usable runtime or device;
    the call itself succeeds). Lets a caller pick the GPU or the host classification
    before it has a context; the reference's CUDA build makes that choice at compile time

@@ -180,6 +180,14 @@ def ref_lib() -> C.CDLL:
         L.ref_read_fit.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int] + [_P(C.c_float)] * 7
         L.ref_group_areas.restype = C.c_int
         L.ref_group_areas.argtypes = [C.c_char_p, C.c_float, _P(C.c_float), C.c_int, _P(C.c_float)]
+        L.ref_ism.restype = C.c_int
+        L.ref_ism.argtypes = [C.c_char_p, C.c_float, _P(C.c_float), C.c_int, _P(C.c_float), C.c_int, C.c_int,
+                              C.c_int, C.c_int, _P(C.c_float), _P(C.c_int), _P(C.c_float), _P(C.c_float),
+                              _P(C.c_float), _P(C.c_int), _P(C.c_int), _P(C.c_float), _P(C.c_float)]
+        L.ref_rt.restype = C.c_int
+        L.ref_rt.argtypes = [C.c_char_p, C.c_float, _P(C.c_float), C.c_int, _P(C.c_float), _P(C.c_float), C.c_int,
+                             _P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_float, C.c_float, C.c_int, C.c_int,
+                             C.c_int, C.c_float, C.c_int, _P(C.c_float), _P(C.c_float)]
         L.ref_t60.restype = C.c_int
         L.ref_t60.argtypes = [C.c_char_p, C.c_float, C.c_char_p, C.c_float, C.c_float, _P(C.c_float),
                               _P(C.c_float), _P(C.c_float), C.c_int]
@@ -243,3 +251,53 @@ def ref_t60(obj_path: str, h: float, mat_dir: str, V: float, c: float, m_air):
         raise RuntimeError("reference OBJ parse failed")
     return T, al[:8 * n].reshape(n, 8).copy()
 
+
+
+def ref_ism(obj_path: str, h: float, src, rec, order: int, cap_img: int = 200000, cap_b: int = 2000000):
+    """The reference's ism_solve: (order_used, direct [ns, nr], images[s][r] = list of
+    (distance, doa[3], amp, groups, cos, angles))."""
+    src = np.ascontiguousarray(src, np.float32).ravel()
+    rec = np.ascontiguousarray(rec, np.float32).ravel()
+    ns, nr = src.size // 3, rec.size // 3
+    direct = np.zeros(ns * nr, np.float32)
+    n_img = np.zeros(ns * nr, np.int32)
+    dist, amp = np.zeros(cap_img, np.float32), np.zeros(cap_img, np.float32)
+    doa = np.zeros(3 * cap_img, np.float32)
+    order_out = np.zeros(cap_img, np.int32)
+    grp, cs, ang = np.zeros(cap_b, np.int32), np.zeros(cap_b, np.float32), np.zeros(cap_b, np.float32)
+    f = lambda a: _p(a, C.c_float)  # noqa: E731
+    i = lambda a: _p(a, C.c_int)  # noqa: E731
+    used = ref_lib().ref_ism(obj_path.encode(), h, f(src), ns, f(rec), nr, order, cap_img, cap_b, f(direct), i(n_img),
+                             f(dist), f(doa), f(amp), i(order_out), i(grp), f(cs), f(ang))
+    if used < 0:
+        raise RuntimeError("reference ism_solve failed: %d" % used)
+    images, q, b = [], 0, 0
+    for s in range(ns):
+        row = []
+        for r in range(nr):
+            lst = []
+            for _ in range(int(n_img[s * nr + r])):
+                m = int(order_out[q])
+                lst.append((float(dist[q]), doa[3 * q:3 * q + 3].copy(), float(amp[q]), grp[b:b + m].copy(),
+                            cs[b:b + m].copy(), ang[b:b + m].copy()))
+                q += 1
+                b += m
+            row.append(lst)
+        images.append(row)
+    return used, direct.reshape(ns, nr), images
+
+
+def ref_rt(obj_path: str, h: float, src, src_p, rec, alpha_gb, m_air, sctr_g, c, fs, Nt, n_rays, max_order, radius,
+           ism_order):
+    """The reference's rt_build + rt_trace_late: (e_spec, e_diff), each [8, nr, Nt]."""
+    src = np.ascontiguousarray(src, np.float32).ravel()
+    rec = np.ascontiguousarray(rec, np.float32).ravel()
+    ns, nr = src.size // 3, rec.size // 3
+    arr = [np.ascontiguousarray(a, np.float32).ravel() for a in (src_p, alpha_gb, m_air, sctr_g)]
+    es, ed = np.zeros(8 * nr * Nt, np.float32), np.zeros(8 * nr * Nt, np.float32)
+    f = lambda a: _p(a, C.c_float)  # noqa: E731
+    rc = ref_lib().ref_rt(obj_path.encode(), h, f(src), ns, f(arr[0]), f(rec), nr, f(arr[1]), f(arr[2]), f(arr[3]), c,
+                          fs, Nt, n_rays, max_order, radius, ism_order, f(es), f(ed))
+    if rc != 0:
+        raise RuntimeError("reference rt failed: %d" % rc)
+    return es.reshape(8, nr, Nt), ed.reshape(8, nr, Nt)
