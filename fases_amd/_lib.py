@@ -130,6 +130,7 @@ SIGNATURES = {
     "fases_ga_energy": (C.c_int, [C.c_void_p, _flt, _flt]),
     "fases_ga_band_irs": (C.c_int, [C.c_void_p, C.c_int, _flt, _flt]),
     "fases_ga_arrivals": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_int), _flt]),
+    "fases_ga_dump_rays": (C.c_int, [C.c_void_p, C.c_int, _flt, C.c_int, C.c_int, C.c_char_p]),
     "fases_pml_tables": (C.c_int, [C.c_void_p, C.c_int, _P(C.c_int32), _dbl, _dbl, _dbl]),
     "fases_batch_run": (C.c_int, [_P(C.c_void_p), C.c_int, _i64, _P(_P(C.c_double)), _P(C.c_double)]),
     "fases_adjoint_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _P(C.c_double)]),

@@ -16,7 +16,6 @@
 #include "../../include/fases_core.h"
 
 #include <hip/hip_runtime.h>
-#include <cfloat>
 #include <string>
 #include <vector>
 
@@ -109,47 +108,9 @@ struct RtDev {
     float m_air[kBands];
 };
 
-__device__ inline bool slab(const F3 &bmin, const F3 &bmax, const F3 &ro, const F3 &inv, float tmin, float tmax) {
-    float t0 = tmin, t1 = tmax, a, b;
-    a = (bmin.x - ro.x) * inv.x; b = (bmax.x - ro.x) * inv.x;
-    if (a > b) { const float s = a; a = b; b = s; }
-    t0 = (t0 < a) ? a : t0; t1 = (b < t1) ? b : t1;
-    a = (bmin.y - ro.y) * inv.y; b = (bmax.y - ro.y) * inv.y;
-    if (a > b) { const float s = a; a = b; b = s; }
-    t0 = (t0 < a) ? a : t0; t1 = (b < t1) ? b : t1;
-    a = (bmin.z - ro.z) * inv.z; b = (bmax.z - ro.z) * inv.z;
-    if (a > b) { const float s = a; a = b; b = s; }
-    t0 = (t0 < a) ? a : t0; t1 = (b < t1) ? b : t1;
-    return t1 >= t0;
-}
-
-// closest triangle hit beyond tmin (the traversal order of the reference, so ties resolve
-// the same way)
-__device__ bool closest_hit(const RtDev &S, const F3 &ro, const F3 &rd, float tmin, float &t_out, int &tri_out) {
-    const F3 inv = f3(1.f / rd.x, 1.f / rd.y, 1.f / rd.z);
-    float best = FLT_MAX;
-    int bi = -1;
-    int stack[kBvhStack];
-    int sp = 0;
-    stack[sp++] = 0;
-    while (sp) {
-        const BvhNode nd = S.nodes[stack[--sp]];
-        if (!slab(nd.bmin, nd.bmax, ro, inv, tmin, best)) continue;
-        if (nd.count > 0) {
-            for (int k = 0; k < nd.count; ++k) {
-                const int p = nd.first + k;
-                float tt;
-                if (ray_tri(ro, rd, S.a[p], S.b[p], S.c[p], tmin, tt) && tt < best) { best = tt; bi = p; }
-            }
-        } else if (sp < kBvhStack - 2) {
-            stack[sp++] = nd.left;
-            stack[sp++] = nd.right;
-        }
-    }
-    if (bi < 0) return false;
-    t_out = best;
-    tri_out = bi;
-    return true;
+__device__ inline bool closest_hit(const RtDev &S, const F3 &ro, const F3 &rd, float tmin, float &t_out,
+                                   int &tri_out) {
+    return fases::ga::closest_hit(S.nodes, S.a, S.b, S.c, ro, rd, tmin, t_out, tri_out);
 }
 
 __device__ inline float seg_dist2(const F3 &q, const F3 &p0, const F3 &p1, float &s_out) {
@@ -172,14 +133,6 @@ __device__ inline F3 lambert(const F3 &N, float u1, float u2) {
     const F3 dir = f3(x * t.x + y * bt.x + z * N.x, x * t.y + y * bt.y + z * N.y, x * t.z + y * bt.z + z * N.z);
     const float dl = sqrtf(len2(dir));
     return (dl > 1e-8f) ? scl(dir, 1.f / dl) : N;
-}
-
-__device__ inline F3 reflected(const F3 &inc, const F3 &N) {
-    const float k = 2.f * dot(inc, N);
-    F3 ref = f3(inc.x - k * N.x, inc.y - k * N.y, inc.z - k * N.z);
-    const float l = sqrtf(len2(ref));
-    if (l > 1e-20f) ref = scl(ref, 1.f / l);
-    return ref;
 }
 
 __global__ void __launch_bounds__(64) k_rt(RtDev S, RtParams P) {

@@ -98,6 +98,61 @@ GA_FN float refl_at(const Refl &r, float cos_theta) {
     return (float)(((double)xc - 1.0) / ((double)xc + 1.0));
 }
 
+// slab test of a box against the ray segment [tmin, tmax] (inv = 1 / direction); the
+// comparisons keep the reference's std::max / std::min order, so a 0 * inf NaN drops out
+GA_FN bool slab(const F3 &bmin, const F3 &bmax, const F3 &ro, const F3 &inv, float tmin, float tmax) {
+    float t0 = tmin, t1 = tmax, a, b;
+    a = (bmin.x - ro.x) * inv.x; b = (bmax.x - ro.x) * inv.x;
+    if (a > b) { const float s = a; a = b; b = s; }
+    t0 = (t0 < a) ? a : t0; t1 = (b < t1) ? b : t1;
+    a = (bmin.y - ro.y) * inv.y; b = (bmax.y - ro.y) * inv.y;
+    if (a > b) { const float s = a; a = b; b = s; }
+    t0 = (t0 < a) ? a : t0; t1 = (b < t1) ? b : t1;
+    a = (bmin.z - ro.z) * inv.z; b = (bmax.z - ro.z) * inv.z;
+    if (a > b) { const float s = a; a = b; b = s; }
+    t0 = (t0 < a) ? a : t0; t1 = (b < t1) ? b : t1;
+    return t1 >= t0;
+}
+
+// closest triangle hit beyond tmin over a BVH (right child popped first, as the
+// reference traverses, so equal distances resolve to the same triangle)
+GA_FN bool closest_hit(const BvhNode *nodes, const F3 *A, const F3 *B, const F3 *C, const F3 &ro, const F3 &rd,
+                       float tmin, float &t_out, int &tri_out) {
+    const F3 inv = f3(1.f / rd.x, 1.f / rd.y, 1.f / rd.z);
+    float best = 3.402823466e+38f;
+    int bi = -1;
+    int stack[kBvhStack];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const BvhNode nd = nodes[stack[--sp]];
+        if (!slab(nd.bmin, nd.bmax, ro, inv, tmin, best)) continue;
+        if (nd.count > 0) {
+            for (int k = 0; k < nd.count; ++k) {
+                const int p = nd.first + k;
+                float tt;
+                if (ray_tri(ro, rd, A[p], B[p], C[p], tmin, tt) && tt < best) { best = tt; bi = p; }
+            }
+        } else if (sp < kBvhStack - 2) {
+            stack[sp++] = nd.left;
+            stack[sp++] = nd.right;
+        }
+    }
+    if (bi < 0) return false;
+    t_out = best;
+    tri_out = bi;
+    return true;
+}
+
+// mirror direction of a unit incident direction about N, renormalised
+GA_FN F3 reflected(const F3 &inc, const F3 &N) {
+    const float k = 2.f * dot(inc, N);
+    F3 ref = f3(inc.x - k * N.x, inc.y - k * N.y, inc.z - k * N.z);
+    const float l = sqrtf(len2(ref));
+    if (l > 1e-20f) ref = scl(ref, 1.f / l);
+    return ref;
+}
+
 // integer hash of the ray tracer (raytrace3D.h rt_sign / rt_u01)
 GA_FN unsigned mix32(unsigned s) {
     s ^= s >> 16; s *= 0x7feb352du; s ^= s >> 15; s *= 0x846ca68bu; s ^= s >> 16;

@@ -904,4 +904,45 @@ int fases_ga_arrivals(const fases_ga *g, int r, int *n, float *buf) {
     return FASES_OK;
 }
 
+int fases_ga_dump_rays(const fases_scene *scene, int n_src, const float *src_xyz, int n_rays, int max_order,
+                       const char *path) {
+    if (!scene || !path || n_src < 0 || (n_src > 0 && !src_xyz) || n_rays < 1 || max_order < 1)
+        return ga_fail(FASES_E_ARG, "fases_ga_dump_rays: bad argument");
+    const Scene &sc = *reinterpret_cast<const Scene *>(scene);
+    const Mesh &M = sc.mesh;
+    std::FILE *f = std::fopen(path, "w");
+    if (!f) return ga_fail(FASES_E_IO, std::string("cannot write ") + path);
+    for (size_t g = 0; g < M.groups.size(); ++g) std::fprintf(f, "# group %d %s\n", (int)g, M.groups[g].c_str());
+    const RtScene S = rt_scene(M);
+    std::vector<F3> srcs;
+    for (int q = 0; q < n_src; ++q) srcs.push_back(f3(src_xyz[3 * q], src_xyz[3 * q + 1], src_xyz[3 * q + 2]));
+    if (srcs.empty()) srcs.push_back(f3(0, 0, 0));
+    if (S.nodes.empty()) { std::fclose(f); return FASES_OK; }
+    // escaped rays get a stub of a quarter of the scene diagonal (order and group -1)
+    const float stub = 0.25f * sqrtf(len2(sub(S.nodes[0].bmax, S.nodes[0].bmin)));
+    auto world = [&](const F3 &p) { return f3(p.x / 1.f - M.offset.x, p.y / 1.f - M.offset.y, p.z / 1.f - M.offset.z); };
+    int uid = 0;
+    for (size_t si = 0; si < srcs.size(); ++si) {
+        const F3 o0 = placed_point(M, &srcs[si].x);
+        for (int ri = 0; ri < n_rays; ++ri, ++uid) {
+            F3 o = o0, d = fib_dir(ri, n_rays);
+            for (int b = 0; b < max_order; ++b) {
+                float th;
+                int tri;
+                const bool hit = closest_hit(S.nodes.data(), S.a.data(), S.b.data(), S.c.data(), o, d, 1e-4f, th, tri);
+                const F3 p = hit ? add(o, scl(d, th)) : add(o, scl(d, stub));
+                const F3 ow = world(o), pw = world(p);
+                std::fprintf(f, "%d %d %.6g %.6g %.6g %.6g %.6g %.6g %d %d\n", (int)si, uid, ow.x, ow.y, ow.z, pw.x, pw.y,
+                             pw.z, hit ? b : -1, hit ? S.group[(size_t)tri] : -1);
+                if (!hit) break;
+                const F3 ref = reflected(d, S.nrm[(size_t)tri]);
+                o = f3(p.x + ref.x * 1e-4f, p.y + ref.y * 1e-4f, p.z + ref.z * 1e-4f);
+                d = ref;
+            }
+        }
+    }
+    std::fclose(f);
+    return FASES_OK;
+}
+
 }  // extern "C"

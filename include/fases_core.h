@@ -367,6 +367,14 @@ int fases_ga_band_irs(const fases_ga *g, int band, float *p_hf, float *p_rt);
  * [3], pressure gain per octave band [8]. */
 int fases_ga_arrivals(const fases_ga *g, int r, int *n, float *buf);
 
+/* Debug trace for the GUI overlay (--dump-rays, raytrace3D.h rt_dump_paths :581-640):
+ * n_rays Fibonacci rays per source (raw coordinates; none given: the origin), purely
+ * specular, up to max_order segments each, written to `path` as "src ray x0 y0 z0 x1 y1
+ * z1 order group" in raw (OBJ) coordinates after one "# group <id> <name>" line per mesh
+ * group; an escaped ray ends in a stub of a quarter of the scene diagonal (order, group -1). */
+int fases_ga_dump_rays(const fases_scene *scene, int n_src, const float *src_xyz, int n_rays, int max_order,
+                       const char *path);
+
 /* Number of HIP devices visible to this process (0 without a usable runtime or device;
    the call itself succeeds). Lets a caller pick the GPU or the host classification
    before it has a context; the reference's CUDA build makes that choice at compile time

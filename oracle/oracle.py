@@ -188,6 +188,8 @@ def ref_lib() -> C.CDLL:
         L.ref_rt.argtypes = [C.c_char_p, C.c_float, _P(C.c_float), C.c_int, _P(C.c_float), _P(C.c_float), C.c_int,
                              _P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_float, C.c_float, C.c_int, C.c_int,
                              C.c_int, C.c_float, C.c_int, _P(C.c_float), _P(C.c_float)]
+        L.ref_dump_rays.restype = C.c_int
+        L.ref_dump_rays.argtypes = [C.c_char_p, C.c_float, _P(C.c_float), C.c_int, C.c_int, C.c_int, C.c_char_p]
         L.ref_t60.restype = C.c_int
         L.ref_t60.argtypes = [C.c_char_p, C.c_float, C.c_char_p, C.c_float, C.c_float, _P(C.c_float),
                               _P(C.c_float), _P(C.c_float), C.c_int]
@@ -301,3 +303,12 @@ def ref_rt(obj_path: str, h: float, src, src_p, rec, alpha_gb, m_air, sctr_g, c,
     if rc != 0:
         raise RuntimeError("reference rt failed: %d" % rc)
     return es.reshape(8, nr, Nt), ed.reshape(8, nr, Nt)
+
+
+def ref_dump_rays(obj_path: str, h: float, src, n_rays: int, max_order: int, out_path: str) -> None:
+    """The reference's rt_dump_paths (--dump-rays) written to out_path."""
+    src = np.ascontiguousarray(src, np.float32).ravel()
+    rc = ref_lib().ref_dump_rays(obj_path.encode(), h, _p(src, C.c_float), src.size // 3, n_rays, max_order,
+                                 out_path.encode())
+    if rc != 0:
+        raise RuntimeError("reference dump_rays failed: %d" % rc)

@@ -117,6 +117,18 @@ int ref_rt(const char *obj, float h, const float *src, int ns, const float *src_
     return 0;
 }
 
+// rt_dump_paths of the scene (main3D.cu:310-328): the --dump-rays file for all sources.
+int ref_dump_rays(const char *obj, float h, const float *src, int ns, int n_rays, int max_order, const char *file) {
+    Scene3D sc;
+    if (!parse_obj3d(obj, &sc, h)) return -1;
+    const RTScene S = rt_build(sc);
+    std::vector<vec3> srcs;
+    for (int q = 0; q < ns; ++q) srcs.push_back(v3(src[3 * q], src[3 * q + 1], src[3 * q + 2]));
+    if (srcs.empty()) srcs.push_back(v3(0, 0, 0));
+    rt_dump_paths(S, srcs, n_rays, max_order, file, &sc.group_names);
+    return 0;
+}
+
 // read_fit_file into float arrays (exits the process on a malformed file, as the reference does).
 void ref_read_fit(const char *path, int m, int K, int L, float *Rinf, float *A, float *zeta, float *B, float *C,
                   float *alpha, float *beta) {

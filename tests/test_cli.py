@@ -112,10 +112,133 @@ def test_receiver_outside_domain_stops(tmp_path):
     assert "Receiver(s) outside domain" in r.stdout
 
 
-def test_geometrical_acoustics_flags_rejected(tmp_path):
-    paths = make_project(tmp_path)
+OCT = (63.0, 125.0, 250.0, 500.0, 1000.0, 2000.0, 4000.0, 8000.0)
+
+
+def set_config_line(paths, index, value):
+    """config.txt line index + 1 (conf_arr[index]) set to value."""
+    cfg = Path(paths["config"])
+    lines = cfg.read_text().splitlines()
+    lines[index] = str(value)
+    cfg.write_text("\n".join(lines) + "\n")
+
+
+def ga_project(tmp, *, order=2, rays=0, device=None, extra=()):
+    """make_project with ISM order (config line 13), ray count (line 15), per-wall .mat
+    absorption and .sctr scattering files; returns (paths, alpha [6, 8], sctr [6])."""
+    ext = tuple(extra) + (("@device %d" % device,) if device is not None else ())
+    paths = make_project(tmp, precision=1, t_max=0.05, extra=ext)
+    set_config_line(paths, 12, order)
+    set_config_line(paths, 14, rays)
+    root = Path(paths["config"]).parent.parent
+    rng = np.random.default_rng(5)
+    alpha = rng.uniform(0.05, 0.5, (6, 8)).astype(np.float32)
+    sctr = np.full(6, 0.01, np.float32)
+    (root / "input" / "mat").mkdir(parents=True, exist_ok=True)
+    (root / "input" / "sctr").mkdir(parents=True, exist_ok=True)
+    for q, g in enumerate(WALLS):
+        (root / "input" / "mat" / (g + ".mat")).write_text("# alpha 63..8k\n" + " ".join("%.9g" % a for a in alpha[q]) + "\n")
+    for q, g in enumerate(WALLS[:3]):              # the rest keep the 0.01 default
+        sctr[q] = np.float32(0.1 * (q + 1))
+        (root / "input" / "sctr" / (g + ".sctr")).write_text("# scattering\n%.9g  # global\n" % sctr[q])
+    return paths, alpha, sctr
+
+
+def ga_reference_run(paths, alpha_by_wall, sctr_by_wall, order, rays, device):
+    """The library's fases_ga_run on the project's scene, sources and receivers, with the
+    tables the CLI derives: .mat absorption and .sctr scattering per scene group, air
+    attenuation at the config's temperature, 2e-5 10^(Lp / 20) source pressure."""
+    import ctypes as C
+    from fases_amd import _lib
+    from fases_amd import ga as GA
+    from fases_amd.solver import Scene
+    wl = P.load_project(**paths)
+    sc = Scene.from_obj(paths["scene"], np.float32(0.02))
+    idx = [WALLS.index(n) for n in sc.group_names]
+    alpha, sctr = alpha_by_wall[idx], sctr_by_wall[idx]
+    m_air = np.zeros(8, np.float32)
+    oct_ = np.array(OCT, np.float32)
+    _lib.check(_lib.load().fases_air_attenuation(C.c_float(np.float32(20.0) + np.float32(273.15)), C.c_float(0.0),
+                                                C.c_float(1.1), oct_.ctypes.data_as(C.POINTER(C.c_float)), 8,
+                                                m_air.ctypes.data_as(C.POINTER(C.c_float))))
+    src = np.array([[0.3, 0.25, 0.24]], np.float32)
+    lp = np.float32(80.0)
+    src_p = np.array([np.float32(2e-5 * float(np.float32(10.0 ** (lp / np.float32(20.0)))))], np.float32)
+    rec = np.array([(0.52, 0.31, 0.22), (0.21, 0.44, 0.3)], np.float32)
+    Nt = int(np.float32(0.05) / np.float32(1.0 / 44100.0))
+    g = GA.run(sc, src, rec, GA.GAParams(device=device, ism_order=order, n_rays=rays, c=wl.params.c, Nt=Nt),
+               src_pressure=src_p, alpha=alpha, sctr=sctr, m_air=m_air)
+    return g, Nt
+
+
+def test_ga_only_writes_band_files_and_keeps_wave_results(tmp_path):
+    """--ga-only on the host (@device -1: image sources without the ray tracer): HIR / HRT
+    per receiver and octave band above the 1 kHz wave band (2, 4, 8 kHz), ARR per receiver;
+    stale HIR files go, the wave-based IR files stay. The files hold what fases_ga_run of
+    the same scene and tables gives (its synthesis is pinned on its own)."""
+    paths, alpha, sctr = ga_project(tmp_path, order=2, device=-1)
+    out = Path(paths["config"]).parent.parent / "output"
+    (out / "IR0_0.bin").write_bytes(b"\1" * 8)
+    (out / "HIR0_0_63.bin").write_bytes(b"\1" * 8)
+    (out / "header.txt").write_text("kept")
     r = run_cli(paths, "--ga-only")
-    assert r.returncode != 0 and "geometrical-acoustics" in r.stderr
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert (out / "IR0_0.bin").read_bytes() == b"\1" * 8
+    assert (out / "header.txt").read_text() == "kept"
+    assert not (out / "HIR0_0_63.bin").exists()
+    assert "[OK] ISM order used: 2 (RT specular starts at order 3)" in r.stdout
+    g, Nt = ga_reference_run(paths, alpha, sctr, 2, 0, -1)
+    written = sorted(p.name for p in out.glob("HIR*.bin"))
+    assert written == sorted("HIR%d_0_%d.bin" % (q, f) for q in range(2) for f in (2000, 4000, 8000))
+    for b in (5, 6, 7):
+        hf, rt = g.band_irs(b)
+        for q in range(2):
+            got = np.fromfile(out / ("HIR%d_0_%d.bin" % (q, OCT[b])), np.float32)
+            got_rt = np.fromfile(out / ("HRT%d_0_%d.bin" % (q, OCT[b])), np.float32)
+            assert got.size == Nt and np.array_equal(got, hf[q])
+            assert np.array_equal(got_rt, rt[q]) and not got_rt.any()
+            assert np.abs(got).max() > 0
+    for q in range(2):
+        raw = np.fromfile(out / ("ARR%d_0.bin" % q), np.uint8)
+        n = int(raw[:4].view(np.int32)[0])
+        arr = raw[4:].view(np.float32).reshape(-1, 12)
+        assert arr.shape[0] == n and np.array_equal(arr, g.arrivals(q))
+        assert n == 1 + len(g.images(0, q))
+
+
+def test_low_frequency_config_runs_only_the_geometrical_part(tmp_path):
+    """A config frequency below the 63 Hz band's lower edge runs no wave-based part: no
+    IR / header files. The seam is still that band's upper edge (89 Hz), so the bands
+    from 125 Hz up are written (main3D.cu:141-150, 1602)."""
+    paths, alpha, sctr = ga_project(tmp_path, order=1, device=-1)
+    set_config_line(paths, 2, 40.0)
+    out = Path(paths["config"]).parent.parent / "output"
+    r = run_cli(paths)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert not list(out.glob("IR*.bin")) and not (out / "header.txt").exists()
+    assert sorted(int(p.stem.split("_")[2]) for p in out.glob("HIR0_0_*.bin")) == [int(f) for f in OCT[1:]]
+
+
+@pytest.mark.skipif(not __import__("oracle.oracle", fromlist=["x"]).ref_available(),
+                    reason="reference checker not built (oracle/_ref)")
+def test_dump_rays_file_equals_the_reference(tmp_path):
+    """--dump-rays N M writes output/ray_paths.txt and exits: the reference's rt_dump_paths
+    file for the same scene and sources, line for line."""
+    from oracle import oracle as O
+    paths = make_project(tmp_path)
+    src = Path(paths["src"])
+    src.write_text(src.read_text() + "0.6 0.4 0.3 75 500\n")
+    out = Path(paths["config"]).parent.parent / "output"
+    r = run_cli(paths, "--dump-rays", "64", "4")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "[OK] dumped 64 ray paths (max order 4) -> output/ray_paths.txt" in r.stdout
+    assert not list(out.glob("IR*.bin"))
+    ref = tmp_path / "ref_paths.txt"
+    O.ref_dump_rays(paths["scene"], np.float32(0.02), [(0.3, 0.25, 0.24), (0.6, 0.4, 0.3)], 64, 4, str(ref))
+    mine = (out / "ray_paths.txt").read_text().splitlines()
+    theirs = ref.read_text().splitlines()
+    assert mine == theirs
+    assert sum(1 for ln in mine if not ln.startswith("#")) >= 2 * 64
 
 
 def test_load_project_matches_reference_derivations(tmp_path):
@@ -269,3 +392,30 @@ def test_cli_gpu_and_host_classification_give_identical_runs(tmp_path):
     host = [(out / ("IR%d_0.bin" % q)).read_bytes() for q in range(2)]
     assert gpu == host and len(gpu[0]) > 0
 
+
+
+@pytest.mark.gpu
+def test_cli_runs_wave_part_then_geometrical_part(tmp_path):
+    """Full CLI run on the GPU: wave-based IR files, then image sources (GPU validity) and
+    the ray tracer. Arrival lists are deterministic and equal the library's; the ray-traced
+    HRT energy per band matches the library's run within the Monte-Carlo spread of two
+    runs (float atomics sum in another order)."""
+    paths, alpha, sctr = ga_project(tmp_path, order=2, rays=4000)
+    out = Path(paths["config"]).parent.parent / "output"
+    print("[case] fases3D wave + GA", flush=True)
+    r = run_cli(paths)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert (out / "IR0_0.bin").exists() and (out / "header.txt").exists()
+    assert "[OK] Ray-tracer and ISM finished" in r.stdout
+    g, Nt = ga_reference_run(paths, alpha, sctr, 2, 4000, 0)
+    for q in range(2):
+        raw = np.fromfile(out / ("ARR%d_0.bin" % q), np.uint8)
+        arr = raw[4:].view(np.float32).reshape(-1, 12)
+        assert np.array_equal(arr, g.arrivals(q))
+    for b in (5, 6, 7):
+        _, rt = g.band_irs(b)
+        for q in range(2):
+            got = np.fromfile(out / ("HRT%d_0_%d.bin" % (q, OCT[b])), np.float32)
+            assert got.size == Nt and np.isfinite(got).all()
+            e_cli, e_lib = float((got.astype(np.float64) ** 2).sum()), float((rt[q].astype(np.float64) ** 2).sum())
+            assert e_lib > 0 and abs(e_cli - e_lib) <= 0.05 * e_lib, (b, q, e_cli, e_lib)
