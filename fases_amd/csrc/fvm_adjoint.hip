@@ -25,11 +25,16 @@
 namespace {
 
 
-// Tiled flux adjoint: waves per block (the inner kAdjRows - 4 rows are outputs) and the
-// z planes per block. 16 rows: one block per CU at 95 VGPRs; 8-row blocks (two per CU,
-// 104 VGPRs) were measured slower (reverse sweep 7.25 s vs 5.89 s on the 192x160x128
+// Tiled flux adjoint: waves per block (the inner rows - 4 rows are outputs) and the z
+// planes per block. FP32: 16 rows, one block per CU at 95 VGPRs; 8-row blocks (two per
+// CU, 104 VGPRs) were measured slower (reverse sweep 7.25 s vs 5.89 s on the 192x160x128
 // room over 3000 steps): their halo rows cost more than the second block recovers.
-constexpr int kAdjRows = 16;
+// FP64: 8 rows (185 VGPRs, nothing spilled). The 16-row FP64 build has 128 VGPRs and
+// spills ~180 of them to scratch, and that build gave gradients that changed from run to
+// run (relative spread 1e-6, single entries off by 1e12 against the per-cell adjoint)
+// although its source is the FP32 kernel's, which is bit-reproducible in any block order,
+// as is the 8-row FP64 build.
+template <typename R> constexpr int kAdjRowsOf = std::is_same<R, double>::value ? 8 : 16;
 constexpr int kAdjTileZ = 32;
 
 // Device buffers of the adjoint, owned by the context (freed with it).
@@ -479,8 +484,11 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
     // flux adjoint: tiled kernel unless the per-cell one was asked for
     const bool tiled = P.adjoint_variant != 1;
     const int atz = kAdjTileZ;
-    const dim3 agrid((ctx->Nx - 6 + adjtile::TX - 1) / adjtile::TX, (ctx->Ny - 6 + kAdjRows - 5) / (kAdjRows - 4),
-                     (ctx->Nz - 6 + atz - 1) / atz);
+    dim3 agrid((ctx->Nx - 6 + adjtile::TX - 1) / adjtile::TX, (ctx->Ny - 6 + kAdjRowsOf<R> - 5) / (kAdjRowsOf<R> - 4),
+               (ctx->Nz - 6 + atz - 1) / atz);
+    // variant 2: the same tiles in the XCD-aware order, on a 1-D grid
+    const int anbx = P.adjoint_variant == 2 ? (int)agrid.x : 0, anby = (int)agrid.y;
+    if (anbx) agrid = dim3(agrid.x * agrid.y * agrid.z);
     for (int c = A.n_ckpt - 1; c >= 0; --c) {
         const long long n0 = A.step0 + (long long)c * A.every;
         const long long len = std::min<long long>(A.every, A.step0 + nsteps - n0);
@@ -562,17 +570,17 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
                 const AdjK<R> K{kp, kc, kr, (R)(t.beta * P.dt / P.h), (R)t.alpha, (R)t.beta};
                 if (tiled) {
                     if (s == 2)
-                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, false, false, true, kAdjRows>), agrid, dim3(64, kAdjRows), 0,
+                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, false, false, true, kAdjRowsOf<R>>), agrid, dim3(64, kAdjRowsOf<R>), 0,
                                            ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w,
-                                           ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz);
+                                           ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz, anbx, anby);
                     else if (s == 1)
-                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, false, true, true, kAdjRows>), agrid, dim3(64, kAdjRows), 0,
+                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, false, true, true, kAdjRowsOf<R>>), agrid, dim3(64, kAdjRowsOf<R>), 0,
                                            ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w,
-                                           ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz);
+                                           ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz, anbx, anby);
                     else
-                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, true, false, false, kAdjRows>), agrid, dim3(64, kAdjRows), 0,
+                        hipLaunchKernelGGL((k_flux_adj_tiled<S, R, true, false, false, kAdjRowsOf<R>>), agrid, dim3(64, kAdjRowsOf<R>), 0,
                                            ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w,
-                                           ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz);
+                                           ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p, Lb.u, Lb.v, Lb.w, K, atz, anbx, anby);
                 } else if (s == 2)
                     hipLaunchKernelGGL((k_flux_adj_cell<S, R, false, false, true>), fgrid, fb, 0, ctx->stream, g,
                                        t.vin, sc, Lo.p, Lo.u, Lo.v, Lo.w, ctx->d_upd, Li.p, Li.u, Li.v, Li.w, Lb.p,

@@ -318,13 +318,22 @@ __global__ void __launch_bounds__(64 * NW)
 k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__restrict__ L0,
                  const R *__restrict__ L1, const R *__restrict__ L2, const R *__restrict__ L3,
                  const uint8_t *__restrict__ upd, R *lin0, R *lin1, R *lin2, R *lin3, R *lb0, R *lb1, R *lb2,
-                 R *lb3, AdjK<R> K, int tz) {
+                 R *lb3, AdjK<R> K, int tz, int nbx, int nby) {
     using namespace adjtile;
     __shared__ AdjLds<R, NW> T;
     typedef Conv<S, R> CV;
     const int lx = threadIdx.x, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int tid = wy * 64 + lx;
-    const int X0 = 3 + TX * blockIdx.x, Y0 = 3 + (NW - 4) * blockIdx.y, k0 = 3 + tz * blockIdx.z;
+    // tile of this block: the grid's own (bx, by, bz), or, on a 1-D grid (nbx > 0), the
+    // XCD-aware order of the forward kernel (each XCD walks a contiguous tile range)
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (nbx > 0) {
+        const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+        bx = (int)(t % (unsigned)nbx);
+        by = (int)((t / (unsigned)nbx) % (unsigned)nby);
+        bz = (int)(t / ((unsigned)nbx * (unsigned)nby));
+    }
+    const int X0 = 3 + TX * bx, Y0 = 3 + (NW - 4) * by, k0 = 3 + tz * bz;
     const int kend = min(k0 + tz, Nz - 3);
     const long long nxy = (long long)Nx * Ny;
     const int x = X0 - 2 + lx, y = Y0 - 2 + wy;

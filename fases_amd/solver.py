@@ -265,7 +265,7 @@ class SolverParams:
     use_graph: bool = False             # reserved, ignored (fases_run launches stage by stage)
     kernel_variant: int = 0             # 0 auto, 1 per-cell (reference order), 2 tiled
     tile_z: int = 0                     # tiled kernel z planes per block; 0 = autotune
-    adjoint_variant: int = 0            # flux adjoint: 0 tiled, 1 per cell
+    adjoint_variant: int = 0            # flux adjoint: 0 tiled, 1 per cell, 2 tiled in XCD-aware order
     tile_rows: int = 0                  # tiled kernel waves per block (8 or 16); 0 = autotune
     source_patch: int = 0               # FP64 radius around sources under FP32/FP16 storage; 0 = 12
 

@@ -75,7 +75,8 @@ typedef struct fases_params {
     int    tile_z;          /* tiled kernel: z planes per block; 0 = autotune at the
                                first fases_run (times 16, 32, 64, 128 on the real grid) */
     int    adjoint_variant; /* flux adjoint of fases_adjoint: 0 tiled (each WENO pair VJP
-                               once), 1 per cell (neighbours' pairs re-evaluated)        */
+                               once), 1 per cell (neighbours' pairs re-evaluated), 2 tiled
+                               in the XCD-aware tile order (same sums as 0)              */
     int    tile_rows;       /* tiled kernel: waves (grid rows) per block, 8 or 16; 0 = autotune
                                together with tile_z                                        */
     int    source_patch;    /* FP32/FP16 storage: Chebyshev radius of the FP64 region around
