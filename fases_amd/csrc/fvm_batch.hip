@@ -53,10 +53,9 @@ k_flux_batch(const FluxJob<S> *__restrict__ jobs, const int *__restrict__ boff, 
     const int pi = phys(in, parity), pb = phys(0, parity), po = phys(out, parity);
     const TileFields<S> F{J.U[pi][0], J.U[pi][1], J.U[pi][2], J.U[pi][3], J.U[pb][0], J.U[pb][1],
                           J.U[pb][2], J.U[pb][3], J.U[po][0], J.U[po][1], J.U[po][2], J.U[po][3]};
-    const PmlSet pml = J.pml;
     const TileK<float> K = J.K;
     const Scl<float> sc = J.sc;
-    flux_tile<S, float, STAGE, kBatchRows>(L, J.Nx, J.Ny, J.Nz, F, J.upd, pml, K, J.dt, sc, J.tz, bx, by, bz);
+    flux_tile<S, float, STAGE, kBatchRows>(L, J.Nx, J.Ny, J.Nz, F, J.upd, &J.pml, K, J.dt, sc, J.tz, bx, by, bz);
 }
 
 template <typename S> struct PatchJob {
@@ -199,7 +198,17 @@ struct Batch {
     fases_ctx *own;                      // error sink and allocation owner (jobs[0])
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
+    std::vector<hipGraphExec_t> execs;   // captured step pairs, one per phase
+    std::vector<hipGraph_t> graphs;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // Every exit, an error return included, drains the stream before anything its queued
+    // replays use (graphs, execs, phase tables) is released.
     ~Batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &x : execs) (void)hipGraphExecDestroy(x);
+        for (auto &g : graphs) (void)hipGraphDestroy(g);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
         for (void *p : allocs) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -360,18 +369,9 @@ int batch_impl(fases_ctx **ctxs, int n, const int64_t *nsteps, double **ir, doub
     std::vector<int> order((size_t)n);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nsteps[a] < nsteps[b]; });
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    std::vector<hipGraphExec_t> execs;
-    std::vector<hipGraph_t> graphs;
-    auto cleanup = [&]() {
-        for (auto &x : execs) (void)hipGraphExecDestroy(x);
-        for (auto &g : graphs) (void)hipGraphDestroy(g);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-    };
-    HIPCHK(hipEventRecord(e0, bt.stream));
+    HIPCHK(hipEventCreate(&bt.e0));
+    HIPCHK(hipEventCreate(&bt.e1));
+    HIPCHK(hipEventRecord(bt.e0, bt.stream));
     long long done = 0;
     size_t first = 0;
     while (first < order.size()) {
@@ -389,7 +389,7 @@ int batch_impl(fases_ctx **ctxs, int n, const int64_t *nsteps, double **ir, doub
             for (size_t q : idx) { jobs.push_back(act[q]); js.push_back(act_steps[q]); }
             Phase<S> ph;
             int rc = build_phase<S>(bt, jobs, js, ph);
-            if (rc) { cleanup(); return rc; }
+            if (rc) return rc;
             int np_j = 0, ns_j = 0, nb_j = 0, nr_j = 0;
             for (fases_ctx *c : jobs) {
                 np_j += c->n_patch > 0; ns_j += c->n_src > 0; nb_j += c->nB > 0; nr_j += c->n_rec > 0;
@@ -404,9 +404,9 @@ int batch_impl(fases_ctx **ctxs, int n, const int64_t *nsteps, double **ir, doub
                 launch_batch_step<S>(bt.stream, ph, np_j, ns_j, nb_j, nr_j, par0, d_n);
                 launch_batch_step<S>(bt.stream, ph, np_j, ns_j, nb_j, nr_j, par0 ^ 1, d_n);
                 HIPCHK(hipStreamEndCapture(bt.stream, &g));
-                graphs.push_back(g);
+                bt.graphs.push_back(g);
                 HIPCHK(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
-                execs.push_back(x);
+                bt.execs.push_back(x);
                 for (long long r = 0; r < len / 2; ++r) HIPCHK(hipGraphLaunch(x, bt.stream));
             }
             if (len & 1) launch_batch_step<S>(bt.stream, ph, np_j, ns_j, nb_j, nr_j, (int)((until - 1) & 1), d_n);
@@ -414,13 +414,12 @@ int batch_impl(fases_ctx **ctxs, int n, const int64_t *nsteps, double **ir, doub
         }
         while (first < order.size() && nsteps[order[first]] <= done) ++first;
     }
-    HIPCHK(hipEventRecord(e1, bt.stream));
-    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventRecord(bt.e1, bt.stream));
+    HIPCHK(hipEventSynchronize(bt.e1));
     HIPCHK(hipGetLastError());
     float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    HIPCHK(hipEventElapsedTime(&ms, bt.e0, bt.e1));
     if (elapsed_ms) *elapsed_ms = ms;
-    cleanup();
     // results and host-side rotation state of every job
     for (int q = 0; q < n; ++q) {
         fases_ctx *c = ctxs[q];

@@ -463,6 +463,8 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
         if ((rc = dalloc(ctx, &ctx->pml.psi_vy, ny_t))) return rc;
         if ((rc = dalloc(ctx, &ctx->pml.psi_pz, nz_t))) return rc;
         if ((rc = dalloc(ctx, &ctx->pml.psi_wz, nz_t))) return rc;
+        // device copy for the tiled kernels, which read it only in their cPML branch
+        if ((rc = upload(ctx, &ctx->d_pml, &ctx->pml, 1))) return rc;
     }
     ctx->n_src = 0; ctx->n_rec = 0; ctx->ir_cap = 0; ctx->step = 0;
     ctx->tz = 0;

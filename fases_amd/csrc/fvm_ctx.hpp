@@ -60,6 +60,7 @@ struct fases_ctx {
     double *phi[3] = {}, *psi1[3] = {}, *psi2[3] = {};   // [0]=phi, [1]=phi_old, [2]=phi_nu; [K or L][nB]
     // PML (device tables in pml, host copies per axis x, y, z)
     PmlSet pml{};
+    PmlSet *d_pml = nullptr;      // device copy of pml (tiled flux kernels)
     std::vector<int> h_pml_band[3];
     std::vector<double> h_pml_kc[3], h_pml_b[3], h_pml_a[3];
     // sources / receivers
@@ -219,13 +220,13 @@ int launch_tiled(fases_ctx *ctx, int stage, const FieldSet<S> &fi, const FieldSe
     const TileFields<S> F{fi.p, fi.u, fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w};
     if (stage == 0)
         hipLaunchKernelGGL((k_flux_tiled<S, R, 0, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
-                           ctx->d_upd, ctx->pml, K, dt, sc, tz, nbx, nby);
+                           ctx->d_upd, ctx->d_pml, K, dt, sc, tz, nbx, nby);
     else if (stage == 1)
         hipLaunchKernelGGL((k_flux_tiled<S, R, 1, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
-                           ctx->d_upd, ctx->pml, K, dt, sc, tz, nbx, nby);
+                           ctx->d_upd, ctx->d_pml, K, dt, sc, tz, nbx, nby);
     else
         hipLaunchKernelGGL((k_flux_tiled<S, R, 2, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
-                           ctx->d_upd, ctx->pml, K, dt, sc, tz, nbx, nby);
+                           ctx->d_upd, ctx->d_pml, K, dt, sc, tz, nbx, nby);
     return FASES_OK;
 }
 

@@ -7,11 +7,14 @@
 //   z faces: register pipeline along the march (face k+1/2 computed once, carried);
 //   x faces: each lane reconstructs both states of its own cell, the right state of
 //            the next cell comes from the neighbouring lane (shuffle);
-//   y faces: both states per cell written to LDS, read by the rows above and below;
-//            the states of the two halo rows j0-1 and j0+NW (p and v) are four extra
-//            reconstructions, one each on waves 0, 1, NW-2, NW-1, i.e. one per SIMD.
-// p, u (x) and p, v (y) of the current plane sit in LDS with a 3-cell halo; p and w
-// along z stay in registers.
+//   y faces: each wave loads its column's rows j-1 and j+1, forms the row's first
+//            differences and its smoothness term once and posts them to LDS, where the
+//            windows of rows j-1 .. j+1 read them; both states per cell go to LDS, read
+//            by the rows above and below. The halo rows' differences and the one state
+//            the block needs of rows j0-1 and j0+NW (p and v) come from four waves,
+//            0, 1, NW-2, NW-1, i.e. one per SIMD.
+// p and u of the current plane's own rows sit in LDS with a 3-column halo; p and w along
+// z stay in registers.
 //
 // WENO5-Z arithmetic (functions3D.cuh:763-977, eps = 1e-10, q = 1) is restated on
 // first differences of the 5-cell window: candidates c + linear combinations of the
@@ -19,6 +22,7 @@
 // the one-sided combinations t, weights d_r (1 + tau/(beta_r + eps)), normalised once.
 #pragma once
 
+#include <climits>
 #include <type_traits>
 
 #include "fvm_device.hpp"
@@ -43,42 +47,60 @@ __device__ inline R weno_q(R s) {
     return fma(k * s, s, R(4e-10));
 }
 
-// Both WENO5-Z states of a cell from the first differences d1..d4 of its 5-cell window
-// (a,b,c,d,e): d1 = b - a, ..., d4 = e - d, the smoothness terms q0 = weno_q(d4 - d3),
-// q1 = weno_q(d3 - d2), q2 = weno_q(d2 - d1) and its centre c:
-//   A = V^-_{i+1/2} (linear weights 3/10, 3/5, 1/10 on the stencils ending at c+2, c+1, c)
-//   B = V^+_{i-1/2} (mirrored weights 1/10, 3/5, 3/10)
-// Every caller goes through this function, so a face state is the same value whether
-// its differences were computed here or carried along an axis.
+// Unnormalised WENO-Z weights of a cell from the first differences d1..d4 of its 5-cell
+// window (a,b,c,d,e): d1 = b - a, ..., d4 = e - d, and the smoothness terms
+// q0 = weno_q(d4 - d3), q1 = weno_q(d3 - d2), q2 = weno_q(d2 - d1).
+//   4 x smoothness indicators: e_r = 13/3 s^2 + t^2 + 4 eps (the weights only see ratios of
+//   beta + eps and |beta_0 - beta_2|, so beta, tau and eps all carry the factor 4);
+//   d_r (1 + tau/e_r) times e0 e1 e2: g_r = (e_r + tau) prod_{s != r} e_s.
+// Reassociations of this arithmetic were measured in FP32 against FP64 on a 4000-step
+// room (IR) and on the eps-level gradient of a 96x80x64 room: g_r = prod e_s tau +
+// e0 e1 e2 (IR error 2.7e-5 instead of 8.0e-6) and denominators sharing 0.1 (g0 + g2) +
+// 0.6 g1 (gradient error 3.8e-3 instead of 3.1e-4) each fail one of the two bars, so
+// both states keep their own sums.
+template <typename R> struct WenoG {
+    R g0, g1, g2;
+};
 template <typename R>
-__device__ inline void weno_core(R d1, R d2, R d3, R d4, R q0, R q1, R q2, R c, R &A, R &B) {
+__device__ inline WenoG<R> weno_g(R d1, R d2, R d3, R d4, R q0, R q1, R q2) {
     const R t0 = fma(R(-3), d3, d4), t1 = d2 + d3, t2 = fma(R(3), d2, -d1);
-    // 4 x smoothness indicators: 13/3 s^2 + t^2 + 4 eps (the weights only see ratios of
-    // beta + eps and |beta_0 - beta_2|, so beta, tau and eps all carry the factor 4)
     const R e0 = fma(t0, t0, q0);
     const R e1 = fma(t1, t1, q1);
     const R e2 = fma(t2, t2, q2);
     const R tau = fabs(e0 - e2);
-    // unnormalised WENO-Z weights d_r (1 + tau/e_r) times e0 e1 e2:
-    // g_r = (e_r + tau) * prod_{s != r} e_s
-    const R g0 = (e0 + tau) * (e1 * e2), g1 = (e1 + tau) * (e0 * e2), g2 = (e2 + tau) * (e0 * e1);
-    // candidate offsets from c, pre-multiplied by the linear weights
-    //   left  (V^- at i+1/2): d = 3/10, 3/5, 1/10 on {1/3,5/6,-1/6}(c,d,e), {-1/6,5/6,1/3}(b,c,d),
-    //                          {1/3,-7/6,11/6}(a,b,c)
-    //   right (V^+ at i-1/2): d = 1/10, 3/5, 3/10 on {11/6,-7/6,1/3}(c,d,e), {1/3,5/6,-1/6}(b,c,d),
-    //                          {-1/6,5/6,1/3}(a,b,c)
+    return {(e0 + tau) * (e1 * e2), (e1 + tau) * (e0 * e2), (e2 + tau) * (e0 * e1)};
+}
+// Candidate offsets from the centre c, pre-multiplied by the linear weights:
+//   left  (V^- at i+1/2): d = 3/10, 3/5, 1/10 on {1/3,5/6,-1/6}(c,d,e), {-1/6,5/6,1/3}(b,c,d),
+//                          {1/3,-7/6,11/6}(a,b,c)
+//   right (V^+ at i-1/2): d = 1/10, 3/5, 3/10 on {11/6,-7/6,1/3}(c,d,e), {1/3,5/6,-1/6}(b,c,d),
+//                          {-1/6,5/6,1/3}(a,b,c)
+template <typename R>
+__device__ inline R weno_left(const WenoG<R> &g, R d1, R d2, R d3, R d4, R c) {
     const R a0 = fma(R(0.2), d3, R(-0.05) * d4);
     const R a1 = fma(R(0.1), d2, R(0.2) * d3);
     const R a2 = fma(R(1.0 / 12.0), d2, R(-1.0 / 30.0) * d1);
+    const R nl = fma(g.g0, a0, fma(g.g1, a1, g.g2 * a2));
+    return fma(nl, fast_rcp(fma(R(0.3), g.g0, fma(R(0.6), g.g1, R(0.1) * g.g2))), c);
+}
+template <typename R>
+__device__ inline R weno_right(const WenoG<R> &g, R d1, R d2, R d3, R d4, R c) {
     const R b0 = fma(R(-1.0 / 12.0), d3, R(1.0 / 30.0) * d4);
     const R b1 = fma(R(-0.2), d2, R(-0.1) * d3);
     const R b2 = fma(R(-0.2), d2, R(0.05) * d1);
-    const R nl = fma(g0, a0, fma(g1, a1, g2 * a2));
-    const R nr = fma(g0, b0, fma(g1, b1, g2 * b2));
-    const R dl = fma(R(0.3), g0, fma(R(0.6), g1, R(0.1) * g2));
-    const R dr = fma(R(0.1), g0, fma(R(0.6), g1, R(0.3) * g2));
-    A = fma(nl, fast_rcp(dl), c);
-    B = fma(nr, fast_rcp(dr), c);
+    const R nr = fma(g.g0, b0, fma(g.g1, b1, g.g2 * b2));
+    return fma(nr, fast_rcp(fma(R(0.1), g.g0, fma(R(0.6), g.g1, R(0.3) * g.g2))), c);
+}
+
+// Both WENO5-Z states of a cell from the differences and smoothness terms of its window
+// and its centre c: A = V^-_{i+1/2}, B = V^+_{i-1/2}. Every caller goes through these
+// functions, so a face state is the same value whether its differences were computed here
+// or carried along an axis, and whether the cell's other state was wanted or not.
+template <typename R>
+__device__ inline void weno_core(R d1, R d2, R d3, R d4, R q0, R q1, R q2, R c, R &A, R &B) {
+    const WenoG<R> g = weno_g(d1, d2, d3, d4, q0, q1, q2);
+    A = weno_left(g, d1, d2, d3, d4, c);
+    B = weno_right(g, d1, d2, d3, d4, c);
 }
 
 // Both WENO5-Z states of the cell whose 5-cell window is (a,b,c,d,e), centre c.
@@ -86,6 +108,20 @@ template <typename R>
 __device__ inline void weno_pair(R a, R b, R c, R d, R e, R &A, R &B) {
     const R d1 = b - a, d2 = c - b, d3 = d - c, d4 = e - d;
     weno_core(d1, d2, d3, d4, weno_q(d4 - d3), weno_q(d3 - d2), weno_q(d2 - d1), c, A, B);
+}
+// Only the left state A = V^-_{i+1/2} (bit-identical to weno_pair's A) ...
+template <typename R>
+__device__ inline R weno_left_of(R a, R b, R c, R d, R e) {
+    const R d1 = b - a, d2 = c - b, d3 = d - c, d4 = e - d;
+    const WenoG<R> g = weno_g(d1, d2, d3, d4, weno_q(d4 - d3), weno_q(d3 - d2), weno_q(d2 - d1));
+    return weno_left(g, d1, d2, d3, d4, c);
+}
+// ... and only the right state B = V^+_{i-1/2}.
+template <typename R>
+__device__ inline R weno_right_of(R a, R b, R c, R d, R e) {
+    const R d1 = b - a, d2 = c - b, d3 = d - c, d4 = e - d;
+    const WenoG<R> g = weno_g(d1, d2, d3, d4, weno_q(d4 - d3), weno_q(d3 - d2), weno_q(d2 - d1));
+    return weno_right(g, d1, d2, d3, d4, c);
 }
 
 // One variable's window along the z march, carried from plane to plane: the centres
@@ -117,12 +153,12 @@ constexpr int LX = 68;          // LDS columns: x0-3 .. x0+64
 }  // namespace tile
 
 template <typename R, int NW> struct TileLds {
-    R p[NW + 6][tile::LX];      // rows j0-3 .. j0+NW+2
+    R p[NW][tile::LX];          // own rows, columns x0-3 .. x0+64
     R u[NW][tile::LX];
-    R v[NW + 6][64];
+    R dy[2][NW + 5][64];        // y differences D[j] = f[j+1] - f[j] of p, v: rows j0-3 .. j0+NW+1
+    R qy[2][NW + 4][64];        // y smoothness terms weno_q(D[j] - D[j-1]): rows j0-2 .. j0+NW+1
     R ys[4][NW + 2][64];        // y states p^-, p^+, v^-, v^+ of rows j0-1 .. j0+NW
 };
-
 // Constants of the tiled kernel as unevaluated sums hi + lo (lo = 0 for FP64): in a
 // lossless resonant room a constant rounded once (dt/h, 1/(2 rho), 1/3) is a systematic
 // bias of the effective sound speed or amplitude that builds up over thousands of steps,
@@ -201,10 +237,9 @@ template <typename S> struct TileFields {
 // (bx, by, bz) of an Nx x Ny x Nz grid.
 template <typename S, typename R, int STAGE, int NW>
 __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, const TileFields<S> &F,
-                                 const uint8_t *__restrict__ upd, const PmlSet &pml, const TileK<R> &K, R dt,
+                                 const uint8_t *__restrict__ upd, const PmlSet *__restrict__ pmlp, const TileK<R> &K, R dt,
                                  const Scl<R> &sc, int tz, int bx, int by, int bz) {
     using namespace tile;
-    constexpr int LY = NW + 6;
     typedef Conv<S, R> CV;
     const S *__restrict__ ip = F.ip, *__restrict__ iu = F.iu, *__restrict__ iv = F.iv, *__restrict__ iw = F.iw;
     const S *__restrict__ bp = F.bp, *__restrict__ bu = F.bu, *__restrict__ bv = F.bv, *__restrict__ bw = F.bw;
@@ -232,11 +267,10 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         PlaneIO<S>::st(plane_rsrc(f + pl, pbytes), cb, CV::st(x, inv_scale));
     };
 
-    // halo items of this thread: items tid, tid + 64 NW, ... of
-    //   p: rows 0-2 and NW+3..NW+5 (all LX columns), columns 0, 1, LX-2, LX-1 of the own rows;
-    //   u: columns 0, 1, LX-2, LX-1 of the own rows;  v: rows 0-2 and NW+3..NW+5
-    constexpr int nP1 = 6 * LX, nP2 = 4 * NW, nU = 4 * NW, nV = 6 * 64;
-    constexpr int NITEM = nP1 + nP2 + nU + nV;
+    // x-halo items of this thread: items tid, tid + 64 NW, ... of columns 0, 1, LX-2, LX-1
+    // of the own rows, p then u
+    constexpr int nP2 = 4 * NW, nU = 4 * NW;
+    constexpr int NITEM = nP2 + nU;
     constexpr int NH = (NITEM + 64 * NW - 1) / (64 * NW);
     bool h_on[NH];
     const S *hsrc[NH];
@@ -246,31 +280,41 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
 #pragma unroll
     for (int q = 0; q < NH; ++q) {
         const int sideX[4] = {0, 1, LX - 2, LX - 1};
-        const int sideY[6] = {0, 1, 2, NW + 3, NW + 4, NW + 5};
         int t = tid + q * 64 * NW;
         int h_kind = -1, h_r = 0, h_c = 0;
-        if (t < nP1) {
-            h_kind = 0; h_r = sideY[t / LX]; h_c = t % LX;
-        } else if ((t -= nP1) < nP2) {
-            h_kind = 0; h_c = sideX[t / NW]; h_r = 3 + t % NW;
+        if (t < nP2) {
+            h_kind = 0; h_c = sideX[t / NW]; h_r = t % NW;
         } else if ((t -= nP2) < nU) {
             h_kind = 1; h_c = sideX[t / NW]; h_r = t % NW;
-        } else if ((t -= nU) < nV) {
-            h_kind = 2; h_r = sideY[t / 64]; h_c = t % 64;
         }
         h_on[q] = h_kind >= 0;
         h_off[q] = 0;
         if (h_kind >= 0) {
-            int gx = (h_kind == 2) ? (x0 - 1 + h_c) : (x0 - 3 + h_c);
-            int gy = (h_kind == 1) ? (j0 + h_r) : (j0 - 3 + h_r);
-            gx = max(0, min(gx, Nx - 1));
-            gy = max(0, min(gy, Ny - 1));
+            const int gx = max(0, min(x0 - 3 + h_c, Nx - 1));
+            const int gy = min(j0 + h_r, Ny - 1);
             h_off[q] = (unsigned)(gx + Nx * gy);
         }
-        hsrc[q] = (h_kind == 0) ? ip : (h_kind == 1 ? iu : iv);
-        hsc[q] = (h_kind == 0) ? sc.s[0] : (h_kind == 1 ? sc.s[1] : sc.s[2]);
-        hdst[q] = (h_kind == 0) ? &L.p[h_r][h_c] : (h_kind == 1 ? &L.u[h_r][h_c] : &L.v[h_r][h_c]);
+        hsrc[q] = (h_kind == 0) ? ip : iu;
+        hsc[q] = (h_kind == 0) ? sc.s[0] : sc.s[1];
+        hdst[q] = (h_kind == 0) ? &L.p[h_r][h_c] : &L.u[h_r][h_c];
     }
+    // y neighbours of the own column: rows j-1 and j+1 of p and v (byte offsets in a plane)
+    const unsigned cbm = (unsigned)(ic + Nx * min(j - 1, Ny - 1)) * (unsigned)sizeof(S);
+    const unsigned cbp = (unsigned)(ic + Nx * min(j + 1, Ny - 1)) * (unsigned)sizeof(S);
+    // y-halo rows: wave 0 / 1 take p / v below the block (rows j0-3 .. j0), waves NW-2 / NW-1
+    // p / v above it (rows j0+NW-1 .. j0+NW+2); one wave per SIMD
+    const int yh_var = (wy == 0 || wy == NW - 2) ? 0 : ((wy == 1 || wy == NW - 1) ? 1 : -1);
+    const bool yh_low = wy <= 1;
+    const R yh_sc = yh_var == 1 ? sc.s[2] : sc.s[0];
+    unsigned yh_off[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = yh_low ? j0 - 3 + q : j0 + NW - 1 + q;
+        yh_off[q] = (unsigned)(ic + Nx * min(r, Ny - 1)) * (unsigned)sizeof(S);
+    }
+    auto LDO = [&](const S *f, long long pl, unsigned off) __device__ {
+        return PlaneIO<S>::ld(plane_rsrc(f + pl, pbytes), off);
+    };
 
     // z pipeline: carried windows of p and w (centres k..k+2, differences, smoothness
     // terms), the newest plane k+3 (pn, wn), the A-state of cell k and the flux through k-1/2
@@ -300,6 +344,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     // plane of work covers their HBM latency.
     constexpr bool kRaw = !std::is_same<S, R>::value;      // storage needs a conversion
     S nu_s, nv_s, nh_s[NH], pend_p, pend_w;
+    S npm, npp, nvm, nvp, nyh[4] = {S(), S(), S(), S()};
     S nb0 = S(), nb1 = S(), nb2 = S(), nb3 = S();
     unsigned nflag;
     {
@@ -316,6 +361,15 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
 #pragma unroll
         for (int q = 0; q < NH; ++q)                       // per-lane field: no uniform base
             nh_s[q] = h_on[q] ? (hsrc[q] + pl)[h_off[q]] : S();
+        npm = LDO(ip, pl, cbm); npp = LDO(ip, pl, cbp);
+        nvm = LDO(iv, pl, cbm); nvp = LDO(iv, pl, cbp);
+        if (yh_var == 0) {                                 // the p / v plane's own descriptor
+#pragma unroll
+            for (int q = 0; q < 4; ++q) nyh[q] = LDO(ip, pl, yh_off[q]);
+        } else if (yh_var == 1) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) nyh[q] = LDO(iv, pl, yh_off[q]);
+        }
         const long long p3 = nxy * (long long)(k0 + 3);
         pend_p = LDR(ip, p3);
         pend_w = LDR(iw, p3);
@@ -324,10 +378,20 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
             wn = CV::ld(pend_w, sc.s[3]);
         }
     }
-    // cPML: whether the tile's columns or rows reach a band (uniform per block); the
-    // plane's z test is added per plane, so tiles clear of every band skip pml_apply
-    const bool pml_xy = pml.on && (x0 < pml.clear0[0] || x0 + TXO > pml.clear1[0] || j0 < pml.clear0[1] ||
-                                   j0 + NW > pml.clear1[1]);
+    // cPML: whether the tile's columns or rows reach a band (uniform per block); planes
+    // k < pz_lo or k >= pz_hi reach one in z, so tiles clear of every band skip pml_apply.
+    // The tables stay in memory (read only in that branch), which keeps the plane loop's
+    // scalar registers for the field planes.
+    int pz_lo = INT_MIN, pz_hi = INT_MAX;           // no plane
+    {
+        const PmlSet &pm = *pmlp;
+        if (pm.on) {
+            const bool pml_xy = x0 < pm.clear0[0] || x0 + TXO > pm.clear1[0] || j0 < pm.clear0[1] ||
+                                j0 + NW > pm.clear1[1];
+            pz_lo = pml_xy ? INT_MAX : pm.clear0[2];    // every plane / the z bands
+            pz_hi = pml_xy ? INT_MIN : pm.clear1[2];
+        }
+    }
 
     // One plane k.
     auto plane = [&](int k) __device__ __attribute__((always_inline)) {
@@ -348,6 +412,11 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         R hv[NH];
 #pragma unroll
         for (int q = 0; q < NH; ++q) hv[q] = h_on[q] ? CV::ld(nh_s[q], hsc[q]) : R(0);
+        const R pm1 = CV::ld(npm, sc.s[0]), pp1 = CV::ld(npp, sc.s[0]);
+        const R vm1 = CV::ld(nvm, sc.s[2]), vp1 = CV::ld(nvp, sc.s[2]);
+        R yh[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) yh[q] = yh_var >= 0 ? CV::ld(nyh[q], yh_sc) : R(0);
         // ---- the next plane's loads (p, w of plane k+4 for the z window)
         auto issue_next = [&]() __device__ __attribute__((always_inline)) {
             if (k + 1 >= kend) return;
@@ -364,47 +433,87 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
 #pragma unroll
             for (int q = 0; q < NH; ++q)
                 if (h_on[q]) nh_s[q] = (hsrc[q] + pn)[h_off[q]];
+            npm = LDO(ip, pn, cbm); npp = LDO(ip, pn, cbp);
+            nvm = LDO(iv, pn, cbm); nvp = LDO(iv, pn, cbp);
+            if (yh_var == 0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) nyh[q] = LDO(ip, pn, yh_off[q]);
+            } else if (yh_var == 1) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) nyh[q] = LDO(iv, pn, yh_off[q]);
+            }
             const long long p4 = nxy * (long long)(k + 4);
             pend_p = LDR(ip, p4);
             pend_w = LDR(iw, p4);
         };
         if constexpr (kRaw) issue_next();
+        // ---- y differences and smoothness terms: the own row's, and on the halo waves
+        // those of the rows beyond the block
+        const R Dpm = pc - pm1, Dp = pp1 - pc, Qp = weno_q(Dp - Dpm);
+        const R Dvm = vc - vm1, Dv = vp1 - vc, Qv = weno_q(Dv - Dvm);
+        R hD0 = R(0), hD1 = R(0), hD2 = R(0), hQ0 = R(0), hQ1 = R(0);
+        if (yh_var >= 0) {
+            hD0 = yh[1] - yh[0];
+            hD1 = yh[2] - yh[1];
+            hD2 = yh[3] - yh[2];
+            hQ0 = weno_q(hD1 - hD0);
+            hQ1 = weno_q(hD2 - hD1);
+            if (yh_low) {        // D[j0-3 .. j0-1], q[j0-2 .. j0-1]
+                L.dy[yh_var][0][lx] = hD0;
+                L.dy[yh_var][1][lx] = hD1;
+                L.dy[yh_var][2][lx] = hD2;
+                L.qy[yh_var][0][lx] = hQ0;
+                L.qy[yh_var][1][lx] = hQ1;
+            } else {             // D[j0+NW .. j0+NW+1], q[j0+NW .. j0+NW+1] (D[j0+NW-1] is wave NW-1's)
+                L.dy[yh_var][NW + 3][lx] = hD1;
+                L.dy[yh_var][NW + 4][lx] = hD2;
+                L.qy[yh_var][NW + 2][lx] = hQ0;
+                L.qy[yh_var][NW + 3][lx] = hQ1;
+            }
+        }
         // ---- fill the plane tiles
-        L.p[wy + 3][lx + 2] = pc;
+        L.p[wy][lx + 2] = pc;
         L.u[wy][lx + 2] = uc;
-        L.v[wy + 3][lx] = vc;
+        L.dy[0][wy + 3][lx] = Dp;
+        L.qy[0][wy + 2][lx] = Qp;
+        L.dy[1][wy + 3][lx] = Dv;
+        L.qy[1][wy + 2][lx] = Qv;
 #pragma unroll
         for (int q = 0; q < NH; ++q)
             if (h_on[q]) *hdst[q] = hv[q];
         __syncthreads();
         if constexpr (!kRaw) issue_next();
-        // ---- y states of the own row, and of the halo rows on four waves
+        // ---- y states of the own row (window rows j-2 .. j+2: D[j-2 .. j+1], q[j-1 .. j+1]),
+        // and of the halo rows on four waves
         {
             R A, B, Av, Bv;
-            weno_pair(L.p[wy + 1][lx + 2], L.p[wy + 2][lx + 2], pc, L.p[wy + 4][lx + 2], L.p[wy + 5][lx + 2], A, B);
-            weno_pair(L.v[wy + 1][lx], L.v[wy + 2][lx], vc, L.v[wy + 4][lx], L.v[wy + 5][lx], Av, Bv);
+            weno_core(L.dy[0][wy + 1][lx], Dpm, Dp, L.dy[0][wy + 4][lx], L.qy[0][wy + 3][lx], Qp,
+                      L.qy[0][wy + 1][lx], pc, A, B);
+            weno_core(L.dy[1][wy + 1][lx], Dvm, Dv, L.dy[1][wy + 4][lx], L.qy[1][wy + 3][lx], Qv,
+                      L.qy[1][wy + 1][lx], vc, Av, Bv);
             L.ys[0][wy + 1][lx] = A;
             L.ys[1][wy + 1][lx] = B;
             L.ys[2][wy + 1][lx] = Av;
             L.ys[3][wy + 1][lx] = Bv;
-            if (wy == 0 || wy == NW - 2) {           // p of row j0-1 (wave 0) / j0+NW (wave NW-2)
-                const int r = wy == 0 ? 0 : NW + 1;  // ys row; its p window is L.p rows r..r+4
-                weno_pair(L.p[r][lx + 2], L.p[r + 1][lx + 2], L.p[r + 2][lx + 2], L.p[r + 3][lx + 2],
-                          L.p[r + 4][lx + 2], A, B);
-                L.ys[0][r][lx] = A;
-                L.ys[1][r][lx] = B;
-            } else if (wy == 1 || wy == NW - 1) {    // v of row j0-1 (wave 1) / j0+NW (wave NW-1)
-                const int r = wy == 1 ? 0 : NW + 1;
-                weno_pair(L.v[r][lx], L.v[r + 1][lx], L.v[r + 2][lx], L.v[r + 3][lx], L.v[r + 4][lx], Av, Bv);
-                L.ys[2][r][lx] = Av;
-                L.ys[3][r][lx] = Bv;
+            // halo rows: the block needs the left state of row j0-1 (face j0-1/2) and the
+            // right state of row j0+NW (face j0+NW-1/2)
+            if (yh_var >= 0) {
+                if (yh_low) {    // row j0-1: D[j0-3 .. j0], q[j0-2 .. j0], centre f[j0-1]
+                    const WenoG<R> g = weno_g(hD0, hD1, hD2, L.dy[yh_var][3][lx], L.qy[yh_var][2][lx], hQ1, hQ0);
+                    L.ys[2 * yh_var][0][lx] =
+                        weno_left(g, hD0, hD1, hD2, L.dy[yh_var][3][lx], yh[2]);
+                } else {         // row j0+NW: D[j0+NW-2 .. j0+NW+1], q[j0+NW-1 .. j0+NW+1], centre f[j0+NW]
+                    const R d1 = L.dy[yh_var][NW + 1][lx];
+                    const WenoG<R> g = weno_g(d1, hD0, hD1, hD2, hQ1, hQ0, L.qy[yh_var][NW + 1][lx]);
+                    L.ys[2 * yh_var + 1][NW + 1][lx] = weno_right(g, d1, hD0, hD1, hD2, yh[1]);
+                }
             }
         }
         // ---- x faces (shuffles are wave-wide: every lane takes part)
         R FxP_p, FxP_u, FxM_p, FxM_u;
         {
             R A, B, Au, Bu;
-            weno_pair(L.p[wy + 3][lx], L.p[wy + 3][lx + 1], pc, L.p[wy + 3][lx + 3], L.p[wy + 3][lx + 4], A, B);
+            weno_pair(L.p[wy][lx], L.p[wy][lx + 1], pc, L.p[wy][lx + 3], L.p[wy][lx + 4], A, B);
             weno_pair(L.u[wy][lx], L.u[wy][lx + 1], uc, L.u[wy][lx + 3], L.u[wy][lx + 4], Au, Bu);
             const R Bn = shfl_next(B), Bun = shfl_next(Bu);
             godunov_f(A, Bn, Au, Bun, K, FxP_p, FxP_u);          // face i+1/2
@@ -450,8 +559,8 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
                     wgt = R(2) / R(3);
                 }
             }
-            if (pml_xy || (pml.on && (k < pml.clear0[2] || k >= pml.clear1[2])))
-                pml_apply<R, STAGE == 2>(pml, i, j, k, Nx, Ny, po, uo, vo, wo, (-FxP_p + FxM_p) * K.ih,
+            if (k < pz_lo || k >= pz_hi)
+                pml_apply<R, STAGE == 2>(*pmlp, i, j, k, Nx, Ny, po, uo, vo, wo, (-FxP_p + FxM_p) * K.ih,
                                          (-FyP_p + FyM_p) * K.ih, (-FzP_p + FzP) * K.ih, Du * K.ih, Dv * K.ih,
                                          Dw * K.ih, wgt * dt);
             ST(ou, pl, uo, sc.is[1]);
@@ -497,7 +606,8 @@ __device__ inline unsigned xcd_tile(unsigned w, unsigned n) {
 
 template <typename S, typename R, int STAGE, int NW>
 __global__ void __launch_bounds__(64 * NW)
-k_flux_tiled(int Nx, int Ny, int Nz, TileFields<S> F, const uint8_t *__restrict__ upd, PmlSet pml, TileK<R> K, R dt,
+k_flux_tiled(int Nx, int Ny, int Nz, TileFields<S> F, const uint8_t *__restrict__ upd, const PmlSet *__restrict__ pml,
+             TileK<R> K, R dt,
              Scl<R> sc, int tz, int nbx, int nby) {
     __shared__ TileLds<R, NW> L;
     const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
