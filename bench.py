@@ -12,9 +12,11 @@ relative L2 vs the FP64 path over a fixed window that holds the direct sound at 
 receiver (independent of --steps), and the FP64 CPU oracle timed on bounded samples on
 this host. Beside the headline:
 the same room with FP16 storage (per-field power-of-two scales, FP32 math) and its IR
-error vs the FP64 and FP32 runs, and the batched benchmark (32 seeded rooms of 0.9-7.9 M
+error vs the FP64 and FP32 runs, the batched benchmark (32 seeded rooms of 0.9-7.9 M
 cells, 1500-4000 steps each, FP32) run as one batch vs the same jobs run one after the
-other.
+other, the adjoint gradient vs the plain forward, the scene classification, and the
+geometrical acoustics (image sources + ray tracer) of a tessellated room next to the
+reference's own ray tracer on the host.
 """
 from __future__ import annotations
 
@@ -110,6 +112,83 @@ def profile_json(name: str):
         return None
 
 
+class stdout_to_stderr:
+    """Sends what C code prints to stdout (file descriptor 1) to stderr meanwhile, so the
+    bench's stdout stays one JSON line."""
+
+    def __enter__(self):
+        import ctypes
+        self._libc = ctypes.CDLL(None)
+        sys.stdout.flush()
+        self._libc.fflush(None)
+        self._saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        self._libc.fflush(None)
+        os.dup2(self._saved, 1)
+        os.close(self._saved)
+        return False
+
+
+def ga_timing(n_rays: int) -> dict:
+    """Geometrical acoustics of a 12 x 9 x 4 m room whose walls are split into 16 x 16
+    quads (3072 triangles, six wall groups): image sources to order 3 (GPU path validity)
+    and the ray tracer (n_rays per source, 2 sources, 4 receivers, 1 s at 44.1 kHz),
+    beside the reference's own rt_trace_late built from its sources (one core) on the
+    same rays, with the echogram energies compared per band and receiver."""
+    import tempfile
+
+    from fases_amd import ga as GA
+    from fases_amd.solver import Scene
+    from fases_amd.workloads import write_tessellated_room
+
+    rng = np.random.default_rng(2024)
+    src = np.array([(2.1, 3.0, 1.5), (9.4, 6.2, 1.2)], np.float32)
+    rec = np.array([(6.0, 4.5, 1.6), (10.5, 1.5, 1.1), (3.2, 7.7, 2.4), (7.9, 2.4, 3.1)], np.float32)
+    alpha = rng.uniform(0.05, 0.4, (6, 8)).astype(np.float32)
+    sctr = rng.uniform(0.05, 0.3, 6).astype(np.float32)
+    m_air = np.array([1e-5, 4e-5, 1.2e-4, 3e-4, 6e-4, 1.1e-3, 2.8e-3, 8.5e-3], np.float32)
+    src_p = np.ones(2, np.float32)
+    c, fs, Nt = 343.0, 44100.0, 44100
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "tess.obj")
+        ntri = write_tessellated_room(path, 12.0, 9.0, 4.0, 16)
+        sc = Scene.from_obj(path, np.float32(0.05))
+        prm = GA.GAParams(device=0, ism_order=3, n_rays=n_rays, c=c, fs=fs, Nt=Nt)
+        GA.run(sc, src[:1], rec[:1], GA.GAParams(device=0, ism_order=1, n_rays=64, c=c, fs=fs, Nt=64),
+               src_pressure=src_p[:1], alpha=alpha, sctr=sctr, m_air=m_air).close()      # warm-up
+        t = time.time()
+        g = GA.run(sc, src, rec, prm, src_pressure=src_p, alpha=alpha, sctr=sctr, m_air=m_air)
+        wall = time.time() - t
+        info = g.info()
+        es, ed = g.energy()
+        g.close()
+        out = {"workload": "12x9x4 m room, walls in 16x16 quads (%d triangles), ISM order 3, %d rays x 2 sources, "
+                           "4 receivers, 1 s at 44.1 kHz" % (ntri, n_rays),
+               "ism_ms": round(info["ism_ms"], 2), "image_paths": info["paths"], "rt_ms": round(info["rt_ms"], 2),
+               "rt_kernel_ms": round(info["rt_kernel_ms"], 2), "wall_s": round(wall, 3),
+               "rays_per_s": round(2 * n_rays / (info["rt_kernel_ms"] * 1e-3), 1)}
+        from oracle import oracle as O
+        if O.ref_available():
+            t = time.time()
+            with stdout_to_stderr():
+                res, red = O.ref_rt(path, np.float32(0.05), src, src_p, rec, alpha, m_air, sctr, c, fs, Nt, n_rays,
+                                    100000, 0.12, info["order_used"])
+            el = time.time() - t
+            mine = (es + ed).sum(axis=2)
+            ref = (res + red).sum(axis=2)
+            out["cpu_reference"] = {"kind": "reference", "cores": 1, "rt_s": round(el, 3),
+                                    "sample": "the reference's rt_build + rt_trace_late (raytrace3D.h) built from "
+                                              "its sources without OpenMP, same scene and rays"}
+            out["rt_speedup_vs_cpu_reference"] = round(el / (info["rt_kernel_ms"] * 1e-3), 1)
+            out["energy_max_rel_diff_vs_reference"] = float(np.abs(mine - ref).max() / ref.max())
+    log("geometrical acoustics: ISM %.1f ms, ray tracer %.1f ms (%s rays/s)%s"
+        % (out["ism_ms"], out["rt_kernel_ms"], out["rays_per_s"],
+           ", reference CPU %.2f s" % out["cpu_reference"]["rt_s"] if "cpu_reference" in out else ""))
+    return out
+
+
 def chunked_run(sv, steps, what, chunk=1000):
     """IR blocks of `steps` steps run in chunks, with a progress line per chunk (the
     FP64 path over thousands of steps of the largest room takes minutes)."""
@@ -140,6 +219,8 @@ def main() -> int:
     ap.add_argument("--tile-z", type=int, default=0, help="flux kernel z depth per block (0: autotune)")
     ap.add_argument("--tile-rows", type=int, default=0, help="flux kernel waves per block, 8 or 16 (0: autotune)")
     ap.add_argument("--no-geometry", action="store_true", help="skip the host vs GPU scene classification timing")
+    ap.add_argument("--ga-rays", type=int, default=10000,
+                    help="rays per source of the geometrical-acoustics benchmark (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the FVM core has no multi-GPU path in this build "
@@ -304,6 +385,8 @@ def main() -> int:
     if not args.no_geometry:
         geometry = geometry_timing(room, wl.h)
 
+    ga = ga_timing(args.ga_rays) if args.ga_rays > 0 else None
+
     cpu = None
     if args.cpu_steps > 0:
         from oracle import oracle as O
@@ -373,6 +456,7 @@ def main() -> int:
         "batched": batch,
         "adjoint": adjoint,
         "geometry": geometry,
+        "geometrical_acoustics": ga,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -605,7 +605,7 @@ int fases_ga_run(const fases_scene *scene, const fases_ga_params *prm, int n_src
             ++ord;
         }
         if (ord < cfg.max_order) {
-            std::printf("\n[!] image-source order reduced %d -> %d (the image tree does not fit in host memory)\n",
+            std::fprintf(stderr, "\n[!] image-source order reduced %d -> %d (the image tree does not fit in host memory)\n",
                         cfg.max_order, ord);
             cfg.max_order = ord;
         }
