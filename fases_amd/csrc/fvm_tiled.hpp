@@ -7,14 +7,11 @@
 //   z faces: register pipeline along the march (face k+1/2 computed once, carried);
 //   x faces: each lane reconstructs both states of its own cell, the right state of
 //            the next cell comes from the neighbouring lane (shuffle);
-//   y faces: each wave loads its column's rows j-1 and j+1, forms the row's first
-//            differences and its smoothness term once and posts them to LDS, where the
-//            windows of rows j-1 .. j+1 read them; both states per cell go to LDS, read
-//            by the rows above and below. The halo rows' differences and the one state
-//            the block needs of rows j0-1 and j0+NW (p and v) come from four waves,
-//            0, 1, NW-2, NW-1, i.e. one per SIMD.
-// p and u of the current plane's own rows sit in LDS with a 3-column halo; p and w along
-// z stay in registers.
+//   y faces: both states per cell written to LDS, read by the rows above and below; the
+//            one state the block needs of each halo row (left of j0-1, right of j0+NW, p
+//            and v) comes from four waves, 0, 1, NW-2, NW-1, i.e. one per SIMD.
+// p, u (x) and p, v (y) of the current plane sit in LDS with a 3-cell halo; p and w
+// along z stay in registers.
 //
 // WENO5-Z arithmetic (functions3D.cuh:763-977, eps = 1e-10, q = 1) is restated on
 // first differences of the 5-cell window: candidates c + linear combinations of the
@@ -153,10 +150,9 @@ constexpr int LX = 68;          // LDS columns: x0-3 .. x0+64
 }  // namespace tile
 
 template <typename R, int NW> struct TileLds {
-    R p[NW][tile::LX];          // own rows, columns x0-3 .. x0+64
+    R p[NW + 6][tile::LX];      // rows j0-3 .. j0+NW+2
     R u[NW][tile::LX];
-    R dy[2][NW + 5][64];        // y differences D[j] = f[j+1] - f[j] of p, v: rows j0-3 .. j0+NW+1
-    R qy[2][NW + 4][64];        // y smoothness terms weno_q(D[j] - D[j-1]): rows j0-2 .. j0+NW+1
+    R v[NW + 6][64];
     R ys[4][NW + 2][64];        // y states p^-, p^+, v^-, v^+ of rows j0-1 .. j0+NW
 };
 // Constants of the tiled kernel as unevaluated sums hi + lo (lo = 0 for FP64): in a
@@ -267,10 +263,11 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         PlaneIO<S>::st(plane_rsrc(f + pl, pbytes), cb, CV::st(x, inv_scale));
     };
 
-    // x-halo items of this thread: items tid, tid + 64 NW, ... of columns 0, 1, LX-2, LX-1
-    // of the own rows, p then u
-    constexpr int nP2 = 4 * NW, nU = 4 * NW;
-    constexpr int NITEM = nP2 + nU;
+    // halo items of this thread: items tid, tid + 64 NW, ... of
+    //   p: rows 0-2 and NW+3..NW+5 (all LX columns), columns 0, 1, LX-2, LX-1 of the own rows;
+    //   u: columns 0, 1, LX-2, LX-1 of the own rows;  v: rows 0-2 and NW+3..NW+5
+    constexpr int nP1 = 6 * LX, nP2 = 4 * NW, nU = 4 * NW, nV = 6 * 64;
+    constexpr int NITEM = nP1 + nP2 + nU + nV;
     constexpr int NH = (NITEM + 64 * NW - 1) / (64 * NW);
     bool h_on[NH];
     const S *hsrc[NH];
@@ -280,41 +277,31 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
 #pragma unroll
     for (int q = 0; q < NH; ++q) {
         const int sideX[4] = {0, 1, LX - 2, LX - 1};
+        const int sideY[6] = {0, 1, 2, NW + 3, NW + 4, NW + 5};
         int t = tid + q * 64 * NW;
         int h_kind = -1, h_r = 0, h_c = 0;
-        if (t < nP2) {
-            h_kind = 0; h_c = sideX[t / NW]; h_r = t % NW;
+        if (t < nP1) {
+            h_kind = 0; h_r = sideY[t / LX]; h_c = t % LX;
+        } else if ((t -= nP1) < nP2) {
+            h_kind = 0; h_c = sideX[t / NW]; h_r = 3 + t % NW;
         } else if ((t -= nP2) < nU) {
             h_kind = 1; h_c = sideX[t / NW]; h_r = t % NW;
+        } else if ((t -= nU) < nV) {
+            h_kind = 2; h_r = sideY[t / 64]; h_c = t % 64;
         }
         h_on[q] = h_kind >= 0;
         h_off[q] = 0;
         if (h_kind >= 0) {
-            const int gx = max(0, min(x0 - 3 + h_c, Nx - 1));
-            const int gy = min(j0 + h_r, Ny - 1);
+            int gx = (h_kind == 2) ? (x0 - 1 + h_c) : (x0 - 3 + h_c);
+            int gy = (h_kind == 1) ? (j0 + h_r) : (j0 - 3 + h_r);
+            gx = max(0, min(gx, Nx - 1));
+            gy = max(0, min(gy, Ny - 1));
             h_off[q] = (unsigned)(gx + Nx * gy);
         }
-        hsrc[q] = (h_kind == 0) ? ip : iu;
-        hsc[q] = (h_kind == 0) ? sc.s[0] : sc.s[1];
-        hdst[q] = (h_kind == 0) ? &L.p[h_r][h_c] : &L.u[h_r][h_c];
+        hsrc[q] = (h_kind == 0) ? ip : (h_kind == 1 ? iu : iv);
+        hsc[q] = (h_kind == 0) ? sc.s[0] : (h_kind == 1 ? sc.s[1] : sc.s[2]);
+        hdst[q] = (h_kind == 0) ? &L.p[h_r][h_c] : (h_kind == 1 ? &L.u[h_r][h_c] : &L.v[h_r][h_c]);
     }
-    // y neighbours of the own column: rows j-1 and j+1 of p and v (byte offsets in a plane)
-    const unsigned cbm = (unsigned)(ic + Nx * min(j - 1, Ny - 1)) * (unsigned)sizeof(S);
-    const unsigned cbp = (unsigned)(ic + Nx * min(j + 1, Ny - 1)) * (unsigned)sizeof(S);
-    // y-halo rows: wave 0 / 1 take p / v below the block (rows j0-3 .. j0), waves NW-2 / NW-1
-    // p / v above it (rows j0+NW-1 .. j0+NW+2); one wave per SIMD
-    const int yh_var = (wy == 0 || wy == NW - 2) ? 0 : ((wy == 1 || wy == NW - 1) ? 1 : -1);
-    const bool yh_low = wy <= 1;
-    const R yh_sc = yh_var == 1 ? sc.s[2] : sc.s[0];
-    unsigned yh_off[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int r = yh_low ? j0 - 3 + q : j0 + NW - 1 + q;
-        yh_off[q] = (unsigned)(ic + Nx * min(r, Ny - 1)) * (unsigned)sizeof(S);
-    }
-    auto LDO = [&](const S *f, long long pl, unsigned off) __device__ {
-        return PlaneIO<S>::ld(plane_rsrc(f + pl, pbytes), off);
-    };
 
     // z pipeline: carried windows of p and w (centres k..k+2, differences, smoothness
     // terms), the newest plane k+3 (pn, wn), the A-state of cell k and the flux through k-1/2
@@ -344,7 +331,6 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     // plane of work covers their HBM latency.
     constexpr bool kRaw = !std::is_same<S, R>::value;      // storage needs a conversion
     S nu_s, nv_s, nh_s[NH], pend_p, pend_w;
-    S npm, npp, nvm, nvp, nyh[4] = {S(), S(), S(), S()};
     S nb0 = S(), nb1 = S(), nb2 = S(), nb3 = S();
     unsigned nflag;
     {
@@ -361,15 +347,6 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
 #pragma unroll
         for (int q = 0; q < NH; ++q)                       // per-lane field: no uniform base
             nh_s[q] = h_on[q] ? (hsrc[q] + pl)[h_off[q]] : S();
-        npm = LDO(ip, pl, cbm); npp = LDO(ip, pl, cbp);
-        nvm = LDO(iv, pl, cbm); nvp = LDO(iv, pl, cbp);
-        if (yh_var == 0) {                                 // the p / v plane's own descriptor
-#pragma unroll
-            for (int q = 0; q < 4; ++q) nyh[q] = LDO(ip, pl, yh_off[q]);
-        } else if (yh_var == 1) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) nyh[q] = LDO(iv, pl, yh_off[q]);
-        }
         const long long p3 = nxy * (long long)(k0 + 3);
         pend_p = LDR(ip, p3);
         pend_w = LDR(iw, p3);
@@ -412,11 +389,6 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         R hv[NH];
 #pragma unroll
         for (int q = 0; q < NH; ++q) hv[q] = h_on[q] ? CV::ld(nh_s[q], hsc[q]) : R(0);
-        const R pm1 = CV::ld(npm, sc.s[0]), pp1 = CV::ld(npp, sc.s[0]);
-        const R vm1 = CV::ld(nvm, sc.s[2]), vp1 = CV::ld(nvp, sc.s[2]);
-        R yh[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) yh[q] = yh_var >= 0 ? CV::ld(nyh[q], yh_sc) : R(0);
         // ---- the next plane's loads (p, w of plane k+4 for the z window)
         auto issue_next = [&]() __device__ __attribute__((always_inline)) {
             if (k + 1 >= kend) return;
@@ -433,87 +405,51 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
 #pragma unroll
             for (int q = 0; q < NH; ++q)
                 if (h_on[q]) nh_s[q] = (hsrc[q] + pn)[h_off[q]];
-            npm = LDO(ip, pn, cbm); npp = LDO(ip, pn, cbp);
-            nvm = LDO(iv, pn, cbm); nvp = LDO(iv, pn, cbp);
-            if (yh_var == 0) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) nyh[q] = LDO(ip, pn, yh_off[q]);
-            } else if (yh_var == 1) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) nyh[q] = LDO(iv, pn, yh_off[q]);
-            }
             const long long p4 = nxy * (long long)(k + 4);
             pend_p = LDR(ip, p4);
             pend_w = LDR(iw, p4);
         };
         if constexpr (kRaw) issue_next();
-        // ---- y differences and smoothness terms: the own row's, and on the halo waves
-        // those of the rows beyond the block
-        const R Dpm = pc - pm1, Dp = pp1 - pc, Qp = weno_q(Dp - Dpm);
-        const R Dvm = vc - vm1, Dv = vp1 - vc, Qv = weno_q(Dv - Dvm);
-        R hD0 = R(0), hD1 = R(0), hD2 = R(0), hQ0 = R(0), hQ1 = R(0);
-        if (yh_var >= 0) {
-            hD0 = yh[1] - yh[0];
-            hD1 = yh[2] - yh[1];
-            hD2 = yh[3] - yh[2];
-            hQ0 = weno_q(hD1 - hD0);
-            hQ1 = weno_q(hD2 - hD1);
-            if (yh_low) {        // D[j0-3 .. j0-1], q[j0-2 .. j0-1]
-                L.dy[yh_var][0][lx] = hD0;
-                L.dy[yh_var][1][lx] = hD1;
-                L.dy[yh_var][2][lx] = hD2;
-                L.qy[yh_var][0][lx] = hQ0;
-                L.qy[yh_var][1][lx] = hQ1;
-            } else {             // D[j0+NW .. j0+NW+1], q[j0+NW .. j0+NW+1] (D[j0+NW-1] is wave NW-1's)
-                L.dy[yh_var][NW + 3][lx] = hD1;
-                L.dy[yh_var][NW + 4][lx] = hD2;
-                L.qy[yh_var][NW + 2][lx] = hQ0;
-                L.qy[yh_var][NW + 3][lx] = hQ1;
-            }
-        }
         // ---- fill the plane tiles
-        L.p[wy][lx + 2] = pc;
+        L.p[wy + 3][lx + 2] = pc;
         L.u[wy][lx + 2] = uc;
-        L.dy[0][wy + 3][lx] = Dp;
-        L.qy[0][wy + 2][lx] = Qp;
-        L.dy[1][wy + 3][lx] = Dv;
-        L.qy[1][wy + 2][lx] = Qv;
+        L.v[wy + 3][lx] = vc;
 #pragma unroll
         for (int q = 0; q < NH; ++q)
             if (h_on[q]) *hdst[q] = hv[q];
         __syncthreads();
         if constexpr (!kRaw) issue_next();
-        // ---- y states of the own row (window rows j-2 .. j+2: D[j-2 .. j+1], q[j-1 .. j+1]),
-        // and of the halo rows on four waves
+        // ---- y states of the own row, and of the halo rows on four waves
         {
             R A, B, Av, Bv;
-            weno_core(L.dy[0][wy + 1][lx], Dpm, Dp, L.dy[0][wy + 4][lx], L.qy[0][wy + 3][lx], Qp,
-                      L.qy[0][wy + 1][lx], pc, A, B);
-            weno_core(L.dy[1][wy + 1][lx], Dvm, Dv, L.dy[1][wy + 4][lx], L.qy[1][wy + 3][lx], Qv,
-                      L.qy[1][wy + 1][lx], vc, Av, Bv);
+            weno_pair(L.p[wy + 1][lx + 2], L.p[wy + 2][lx + 2], pc, L.p[wy + 4][lx + 2], L.p[wy + 5][lx + 2], A, B);
+            weno_pair(L.v[wy + 1][lx], L.v[wy + 2][lx], vc, L.v[wy + 4][lx], L.v[wy + 5][lx], Av, Bv);
             L.ys[0][wy + 1][lx] = A;
             L.ys[1][wy + 1][lx] = B;
             L.ys[2][wy + 1][lx] = Av;
             L.ys[3][wy + 1][lx] = Bv;
             // halo rows: the block needs the left state of row j0-1 (face j0-1/2) and the
-            // right state of row j0+NW (face j0+NW-1/2)
-            if (yh_var >= 0) {
-                if (yh_low) {    // row j0-1: D[j0-3 .. j0], q[j0-2 .. j0], centre f[j0-1]
-                    const WenoG<R> g = weno_g(hD0, hD1, hD2, L.dy[yh_var][3][lx], L.qy[yh_var][2][lx], hQ1, hQ0);
-                    L.ys[2 * yh_var][0][lx] =
-                        weno_left(g, hD0, hD1, hD2, L.dy[yh_var][3][lx], yh[2]);
-                } else {         // row j0+NW: D[j0+NW-2 .. j0+NW+1], q[j0+NW-1 .. j0+NW+1], centre f[j0+NW]
-                    const R d1 = L.dy[yh_var][NW + 1][lx];
-                    const WenoG<R> g = weno_g(d1, hD0, hD1, hD2, hQ1, hQ0, L.qy[yh_var][NW + 1][lx]);
-                    L.ys[2 * yh_var + 1][NW + 1][lx] = weno_right(g, d1, hD0, hD1, hD2, yh[1]);
-                }
+            // right state of row j0+NW (face j0+NW-1/2), one reconstruction per SIMD
+            if (wy == 0) {                           // p of row j0-1 (ys row 0, L.p rows 0..4)
+                L.ys[0][0][lx] = weno_left_of(L.p[0][lx + 2], L.p[1][lx + 2], L.p[2][lx + 2], L.p[3][lx + 2],
+                                              L.p[4][lx + 2]);
+            } else if (wy == 1) {                    // v of row j0-1
+                L.ys[2][0][lx] = weno_left_of(L.v[0][lx], L.v[1][lx], L.v[2][lx], L.v[3][lx], L.v[4][lx]);
+            } else if (wy == NW - 2) {               // p of row j0+NW (ys row NW+1, L.p rows NW+1..NW+5)
+                constexpr int r = NW + 1;
+                L.ys[1][r][lx] = weno_right_of(L.p[r][lx + 2], L.p[r + 1][lx + 2], L.p[r + 2][lx + 2],
+                                               L.p[r + 3][lx + 2], L.p[r + 4][lx + 2]);
+            } else if (wy == NW - 1) {               // v of row j0+NW
+                constexpr int r = NW + 1;
+                L.ys[3][r][lx] = weno_right_of(L.v[r][lx], L.v[r + 1][lx], L.v[r + 2][lx], L.v[r + 3][lx],
+                                               L.v[r + 4][lx]);
             }
         }
         // ---- x faces (shuffles are wave-wide: every lane takes part)
         R FxP_p, FxP_u, FxM_p, FxM_u;
         {
             R A, B, Au, Bu;
-            weno_pair(L.p[wy][lx], L.p[wy][lx + 1], pc, L.p[wy][lx + 3], L.p[wy][lx + 4], A, B);
+            weno_pair(L.p[wy + 3][lx], L.p[wy + 3][lx + 1], pc, L.p[wy + 3][lx + 3], L.p[wy + 3][lx + 4], A, B);
             weno_pair(L.u[wy][lx], L.u[wy][lx + 1], uc, L.u[wy][lx + 3], L.u[wy][lx + 4], Au, Bu);
             const R Bn = shfl_next(B), Bun = shfl_next(Bu);
             godunov_f(A, Bn, Au, Bun, K, FxP_p, FxP_u);          // face i+1/2
