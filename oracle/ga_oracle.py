@@ -11,7 +11,16 @@ fases_ga_band_irs / fases_ga_arrivals:
 
 Float32 / float64 promotions follow the C++ expressions (float operands, double literals);
 libm's cos / sin / exp of float arguments are evaluated in double and rounded once to
-float32, which is within an ulp of the C library's float versions."""
+float32, which is within an ulp of the C library's float versions.
+
+Pinned (tests/test_ga.py): r_oblique and rt_sign bit for bit against the reference's own
+R_oblique / rt_sign built from its sources (oracle/_ref); r_oblique's closed forms
+(sqrt(1 - alpha) at normal incidence, -1 at grazing, anechoic and rigid limits,
+monotone in the angle); the deposit's closed forms (one sample at an integer delay,
+mirror-symmetric taps at a half-sample delay, unit sum); the direct-sound gain
+p / d exp(-m d), a normally incident image's p amp sqrt(1 - alpha) sqrt(1 - s) / d
+exp(-m d), and the ray-traced 4 pi sign sqrt(e). The arrival list is the same gains and
+delays and is pinned through them."""
 from __future__ import annotations
 
 import math

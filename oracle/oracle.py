@@ -188,6 +188,10 @@ def ref_lib() -> C.CDLL:
         L.ref_rt.argtypes = [C.c_char_p, C.c_float, _P(C.c_float), C.c_int, _P(C.c_float), _P(C.c_float), C.c_int,
                              _P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_float, C.c_float, C.c_int, C.c_int,
                              C.c_int, C.c_float, C.c_int, _P(C.c_float), _P(C.c_float)]
+        L.ref_r_oblique.restype = C.c_float
+        L.ref_r_oblique.argtypes = [C.c_float, C.c_float]
+        L.ref_rt_sign.restype = C.c_float
+        L.ref_rt_sign.argtypes = [C.c_uint]
         L.ref_dump_rays.restype = C.c_int
         L.ref_dump_rays.argtypes = [C.c_char_p, C.c_float, _P(C.c_float), C.c_int, C.c_int, C.c_int, C.c_char_p]
         L.ref_t60.restype = C.c_int

@@ -129,6 +129,11 @@ int ref_dump_rays(const char *obj, float h, const float *src, int ns, int n_rays
     return 0;
 }
 
+// The reference's oblique-incidence reflection factor (parsing3D.h:1178-1186) and the
+// ray tracer's hash sign (raytrace3D.h:69-72), for the IR-synthesis restatement's pins.
+float ref_r_oblique(float alpha, float cos_theta) { return R_oblique(alpha, cos_theta); }
+float ref_rt_sign(unsigned s) { return rt_sign(s); }
+
 // read_fit_file into float arrays (exits the process on a malformed file, as the reference does).
 void ref_read_fit(const char *path, int m, int K, int L, float *Rinf, float *A, float *zeta, float *B, float *C,
                   float *alpha, float *beta) {

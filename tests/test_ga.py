@@ -292,3 +292,78 @@ def test_band_irs_with_rays_match_the_synthesis_restatement(tmp_path):
     g = GA.run(sc, src, rec, GA.GAParams(device=0, ism_order=2, n_rays=n_rays, c=c, fs=fs, Nt=Nt),
                src_pressure=src_p, alpha=alpha, sctr=sctr, m_air=m_air)
     check_band_irs(g, sc, src, rec, src_p, alpha, sctr, m_air, c, fs, Nt, bands=[2, 6], rtol=2e-6)
+
+
+# ------------------------------------------------------------------------------------
+# Pins of the IR-synthesis restatement (oracle/ga_oracle.py) on its own
+# ------------------------------------------------------------------------------------
+
+@needs_ref
+def test_synthesis_helpers_equal_the_reference_build():
+    """R_oblique (parsing3D.h:1178) and rt_sign (raytrace3D.h:69) of the restatement
+    against the reference's own functions built from its sources, bit for bit."""
+    L = O.ref_lib()
+    rng = np.random.default_rng(9)
+    al = np.concatenate([rng.uniform(0, 1, 400), [0.0, 0.99, 0.9900001, 1.0, 1e-6, 2e-5]]).astype(np.float32)
+    cs = np.concatenate([rng.uniform(-1, 1, 400), [1.0, 0.0, 0.5, -1.0, 1e-3, 0.999]]).astype(np.float32)
+    for a, c in zip(al, cs):
+        assert np.float32(GO.r_oblique(a, c)).view(np.uint32) == np.float32(L.ref_r_oblique(a, c)).view(np.uint32), (a, c)
+    for s in list(rng.integers(0, 2 ** 32, 2000, dtype=np.uint64)) + [0, 1, 2 ** 32 - 1, 2654435761]:
+        assert GO.rt_sign(int(s)) == L.ref_rt_sign(int(s)), s
+
+
+def test_oblique_reflection_closed_forms():
+    """A locally reacting surface matched to the random-incidence alpha reflects
+    sqrt(1 - alpha) at normal incidence and -1 at grazing incidence; alpha > 0.99 is
+    anechoic and alpha -> 0 rigid."""
+    for a in (0.02, 0.1, 0.3, 0.55, 0.8, 0.95):
+        assert GO.r_oblique(a, 1.0) == pytest.approx(np.sqrt(1 - a), rel=2e-6)
+        assert GO.r_oblique(a, 0.0) == -1.0
+        # R rises monotonically from -1 (grazing) to sqrt(1 - a) (normal)
+        r = [float(GO.r_oblique(a, c)) for c in np.linspace(0, 1, 21)]
+        assert all(x < y for x, y in zip(r, r[1:]))
+    assert GO.r_oblique(0.995, 0.7) == 0.0
+    assert GO.r_oblique(0.0, 0.3) == 1.0
+
+
+def test_fractional_delay_deposit_closed_forms():
+    """The windowed-sinc deposit of one arrival: at an integer sample delay only that
+    sample receives the gain (sinc zeros at the other integers, up to the rounding of
+    sinf at float multiples of pi); at a half-sample delay
+    the taps are symmetric about it and sum to the gain within the window's DC error;
+    direct sound of a source at distance d arrives with gain p / d exp(-m d)."""
+    Nt = 200
+    row = np.zeros(Nt, np.float32)
+    GO._deposit(row, np.float32(50.0), np.float32(0.7), Nt)
+    assert row[50] == np.float32(0.7) and np.abs(np.delete(row, 50)).max() < 1e-6 * 0.7
+    row = np.zeros(Nt, np.float32)
+    GO._deposit(row, np.float32(80.5), np.float32(1.0), Nt)
+    # taps 64 .. 96 around int(80.5) = 80: n and 161 - n mirror each other about 80.5
+    np.testing.assert_allclose(row[65:81], row[96:80:-1], rtol=1e-5, atol=1e-9)
+    assert abs(float(row.sum()) - 1.0) < 0.02 and np.argmax(row) in (80, 81)
+    # direct sound only: c = fs so the delay in samples is the distance in metres
+    c = fs = 1000.0
+    d = 37.0
+    direct = np.array([[d]], np.float32)
+    m_air = np.full(8, 1e-3, np.float32)
+    zero = np.zeros((8, 1, Nt), np.float32)
+    hf, rt = GO.band_irs(direct, [[[]]], np.array([2.0], np.float32), zero, zero, np.zeros(8, np.float32),
+                         np.zeros(1, np.float32), 1, m_air, c, fs, Nt, 3)
+    g = 2.0 / d * np.exp(-1e-3 * d)
+    assert hf[0, 37] == pytest.approx(g, rel=1e-6) and np.abs(np.delete(hf[0], 37)).max() < 1e-6 * g
+    assert not rt.any()
+    # one normally incident first-order image: gain p amp sqrt(1 - alpha) sqrt(1 - s) / d exp(-m d)
+    im = {"distance": 61.0, "amp": 1.0, "group": np.array([0]), "angle": np.array([0.0], np.float32),
+          "doa": np.zeros(3, np.float32)}
+    alpha = np.full(8, 0.36, np.float32)
+    hf, _ = GO.band_irs(np.array([[0.0]], np.float32), [[[im]]], np.array([1.0], np.float32), zero, zero, alpha,
+                        np.array([0.19], np.float32), 1, m_air, c, fs, Nt, 5)
+    g = 0.8 * 0.9 / 61.0 * np.exp(-1e-3 * 61.0)
+    assert hf[0, 61] == pytest.approx(g, rel=2e-6) and np.abs(np.delete(hf[0], 61)).max() < 1e-6 * g
+    # the ray-traced energy enters as 4 pi sign(t) sqrt(e) at its sample
+    es = zero.copy()
+    es[5, 0, 120] = 0.25
+    hf, rt = GO.band_irs(np.array([[0.0]], np.float32), [[[]]], np.array([1.0], np.float32), es, zero, alpha,
+                         np.zeros(1, np.float32), 1, m_air, c, fs, Nt, 5)
+    assert abs(rt[0, 120]) == pytest.approx(4 * np.pi * 0.5, rel=1e-6) and np.count_nonzero(rt) == 1
+    assert np.array_equal(hf, rt)
