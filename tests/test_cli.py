@@ -186,7 +186,7 @@ def test_ga_only_writes_band_files_and_keeps_wave_results(tmp_path):
     assert (out / "IR0_0.bin").read_bytes() == b"\1" * 8
     assert (out / "header.txt").read_text() == "kept"
     assert not (out / "HIR0_0_63.bin").exists()
-    assert "[OK] ISM order used: 2 (RT specular starts at order 3)" in r.stdout
+    assert "[GA] image-source paths cover orders 0 .. 2; specular ray paths start at order 3" in r.stdout
     g, Nt = ga_reference_run(paths, alpha, sctr, 2, 0, -1)
     written = sorted(p.name for p in out.glob("HIR*.bin"))
     assert written == sorted("HIR%d_0_%d.bin" % (q, f) for q in range(2) for f in (2000, 4000, 8000))
@@ -231,7 +231,7 @@ def test_dump_rays_file_equals_the_reference(tmp_path):
     out = Path(paths["config"]).parent.parent / "output"
     r = run_cli(paths, "--dump-rays", "64", "4")
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    assert "[OK] dumped 64 ray paths (max order 4) -> output/ray_paths.txt" in r.stdout
+    assert "[OK] output/ray_paths.txt: 64 rays per source, at most 4 reflections each" in r.stdout
     assert not list(out.glob("IR*.bin"))
     ref = tmp_path / "ref_paths.txt"
     O.ref_dump_rays(paths["scene"], np.float32(0.02), [(0.3, 0.25, 0.24), (0.6, 0.4, 0.3)], 64, 4, str(ref))
@@ -406,7 +406,7 @@ def test_cli_runs_wave_part_then_geometrical_part(tmp_path):
     r = run_cli(paths)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert (out / "IR0_0.bin").exists() and (out / "header.txt").exists()
-    assert "[OK] Ray-tracer and ISM finished" in r.stdout
+    assert "[OK] HIR / HRT / ARR files written" in r.stdout
     g, Nt = ga_reference_run(paths, alpha, sctr, 2, 4000, 0)
     for q in range(2):
         raw = np.fromfile(out / ("ARR%d_0.bin" % q), np.uint8)
