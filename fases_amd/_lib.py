@@ -92,6 +92,8 @@ SIGNATURES = {
     "fases_scene_load_obj": (C.c_int, [C.c_char_p, C.c_float, _P(C.c_void_p)]),
     "fases_scene_shoebox": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float, _P(C.c_void_p)]),
     "fases_scene_load_obj_dev": (C.c_int, [C.c_char_p, C.c_float, C.c_int, _P(C.c_void_p)]),
+    "fases_scene_load_obj_classified": (C.c_int, [C.c_char_p, C.c_float, _flt, _P(C.c_uint8), C.c_int, C.c_int,
+                                                  C.c_int, _P(C.c_void_p)]),
     "fases_scene_shoebox_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, _P(C.c_void_p)]),
     "fases_scene_info": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int), _P(C.c_int), _flt]),
     "fases_scene_group_name": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int]),

@@ -37,6 +37,11 @@ struct Scene {
     float Lx = 0.f, Ly = 0.f, Lz = 0.f;   // padded extent
     std::vector<float> S;
     std::vector<uint8_t> oid;
+    // a classification supplied by the caller (a cached S / oid): scene_build places the
+    // mesh, checks the grid and adopts these instead of classifying
+    const float *adopt_S = nullptr;
+    const uint8_t *adopt_oid = nullptr;
+    int adopt_dims[3] = {0, 0, 0};
 };
 
 bool mesh_from_obj(const std::string &path, Mesh &m, std::string &err);

@@ -224,6 +224,19 @@ bool scene_build(Scene &sc, std::string &err, int device, int *code) {
     sc.Nx = Nx; sc.Ny = Ny; sc.Nz = Nz;
     const size_t N = (size_t)Nx * Ny * Nz;
     const size_t nxy = (size_t)Nx * Ny;
+    if (sc.adopt_S) {                 // classification supplied (scene cache)
+        if (sc.adopt_dims[0] != Nx || sc.adopt_dims[1] != Ny || sc.adopt_dims[2] != Nz) {
+            err = "supplied classification is for a " + std::to_string(sc.adopt_dims[0]) + "x" +
+                  std::to_string(sc.adopt_dims[1]) + "x" + std::to_string(sc.adopt_dims[2]) + " grid, the scene gives " +
+                  std::to_string(Nx) + "x" + std::to_string(Ny) + "x" + std::to_string(Nz);
+            return false;
+        }
+        sc.S.assign(sc.adopt_S, sc.adopt_S + N);
+        sc.oid.assign(sc.adopt_oid, sc.adopt_oid + N);
+        sc.adopt_S = nullptr;
+        sc.adopt_oid = nullptr;
+        return true;
+    }
 
     // Placed triangle corners, once.
     const size_t nt = m.tris.size();
@@ -392,6 +405,34 @@ int fases_scene_load_obj_dev(const char *path, float h, int device, fases_scene 
     try {
         int code = FASES_E_ARG;
         if (!scene_build(*sc, err, device, &code)) { set_global_error(err); delete sc; return code; }
+    } catch (const std::bad_alloc &) {
+        set_global_error("out of host memory building the scene"); delete sc; return FASES_E_NOMEM;
+    }
+    *out = reinterpret_cast<fases_scene *>(sc);
+    return FASES_OK;
+}
+
+int fases_scene_load_obj_classified(const char *path, float h, const float *S, const uint8_t *oid, int Nx, int Ny,
+                                    int Nz, fases_scene **out) {
+    if (!path || !out || !S || !oid || Nx < 1 || Ny < 1 || Nz < 1) {
+        set_global_error("fases_scene_load_obj_classified: bad argument");
+        return FASES_E_ARG;
+    }
+    *out = nullptr;
+    Scene *sc = new (std::nothrow) Scene();
+    if (!sc) return FASES_E_NOMEM;
+    std::string err;
+    if (!mesh_from_obj(path, sc->mesh, err)) {
+        set_global_error(err); delete sc;
+        return (err.rfind("cannot open", 0) == 0) ? FASES_E_IO : FASES_E_PARSE;
+    }
+    sc->h = h;
+    sc->adopt_S = S;
+    sc->adopt_oid = oid;
+    sc->adopt_dims[0] = Nx; sc->adopt_dims[1] = Ny; sc->adopt_dims[2] = Nz;
+    try {
+        int code = FASES_E_ARG;
+        if (!scene_build(*sc, err, -1, &code)) { set_global_error(err); delete sc; return code; }
     } catch (const std::bad_alloc &) {
         set_global_error("out of host memory building the scene"); delete sc; return FASES_E_NOMEM;
     }

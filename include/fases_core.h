@@ -149,6 +149,14 @@ int fases_scene_shoebox(int nx, int ny, int nz, float h, fases_scene **out);
 int fases_scene_load_obj_dev(const char *path, float h, int device, fases_scene **out);
 int fases_scene_shoebox_dev(int nx, int ny, int nz, float h, int device, fases_scene **out);
 
+/* The OBJ scene with a classification the caller already holds (S float32 and oid, Nx*Ny*Nz
+ * cells, x fastest): parsed and placed as fases_scene_load_obj, the grid checked against
+ * Nx, Ny, Nz (FASES_E_ARG if it differs), S and oid copied in instead of computed. Replaces
+ * the scene-cache reuse of 3D-solver/core/main3D.cu:334-379 (S.bin / oid.bin of a run whose
+ * scene_cache.txt key matches). */
+int fases_scene_load_obj_classified(const char *path, float h, const float *S, const uint8_t *oid, int Nx, int Ny,
+                                    int Nz, fases_scene **out);
+
 /* Padded grid dims, number of mesh groups (materials), padded extent [m]. */
 int fases_scene_info(const fases_scene *s, int *Nx, int *Ny, int *Nz, int *n_groups, float *L3);
 

@@ -419,3 +419,43 @@ def test_cli_runs_wave_part_then_geometrical_part(tmp_path):
             assert got.size == Nt and np.isfinite(got).all()
             e_cli, e_lib = float((got.astype(np.float64) ** 2).sum()), float((rt[q].astype(np.float64) ** 2).sum())
             assert e_lib > 0 and abs(e_cli - e_lib) <= 0.05 * e_lib, (b, q, e_cli, e_lib)
+
+
+def test_scene_cache_written_reused_and_invalidated(tmp_path):
+    """output/S.bin, oid.bin and scene_cache.txt (main3D.cu:334-428): a real run writes the
+    classification and the key (scene path, scene file time, h, tol = 3h) with the grid;
+    the next run reads S and oid from the cache (an edit of S.bin shows up in Vol.bin);
+    touching the scene invalidates it; --estimate reads but never writes it."""
+    import os
+    from fases_amd.solver import Scene
+    paths, _, _ = ga_project(tmp_path, order=0, device=-1)
+    out = Path(paths["config"]).parent.parent / "output"
+    r = run_cli(paths, "--estimate")
+    assert r.returncode == 0, r.stderr
+    assert not (out / "scene_cache.txt").exists() and not (out / "S.bin").exists()
+    r = run_cli(paths, "--ga-only")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "[scene] classifying the scene" in r.stdout
+    sc = Scene.from_obj(paths["scene"], np.float32(0.02))
+    Nx, Ny, Nz = sc.dims
+    S = np.fromfile(out / "S.bin", np.float32)
+    oid = np.fromfile(out / "oid.bin", np.uint8)
+    assert np.array_equal(S.view(np.uint32), sc.S.ravel().view(np.uint32)) and np.array_equal(oid, sc.oid.ravel())
+    key, dims = (out / "scene_cache.txt").read_text().splitlines()
+    assert key.startswith(str(Path(paths["scene"]).resolve()) + " ") and key.endswith(" 0.02 0.06")
+    assert [int(v) for v in dims.split()[:3]] == [Nx, Ny, Nz]
+    # reuse: make a layer of air cells solid in the cached S; the volume follows the cache
+    S2 = S.copy()
+    air = np.flatnonzero(S2 > 0)[:500]
+    S2[air] = -1.0
+    S2.tofile(out / "S.bin")
+    r = run_cli(paths, "--estimate")
+    assert r.returncode == 0 and "read from output/S.bin" in r.stdout, r.stdout[-2000:]
+    h = np.float32(0.02)
+    vol = np.fromfile(Path(paths["config"]).parent / "Vol.bin", np.float32)[0]
+    assert vol == ((np.float32((S2 > 0).sum()) * h) * h) * h
+    # a newer scene file invalidates the key
+    st = os.stat(paths["scene"])
+    os.utime(paths["scene"], ns=(st.st_atime_ns, st.st_mtime_ns + 10 ** 9))
+    r = run_cli(paths, "--estimate")
+    assert r.returncode == 0 and "[scene] classifying the scene" in r.stdout
