@@ -55,7 +55,8 @@ k_flux_batch(const FluxJob<S> *__restrict__ jobs, const int *__restrict__ boff, 
                           J.U[pb][2], J.U[pb][3], J.U[po][0], J.U[po][1], J.U[po][2], J.U[po][3]};
     const TileK<float> K = J.K;
     const Scl<float> sc = J.sc;
-    flux_tile<S, float, STAGE, kBatchRows>(L, J.Nx, J.Ny, J.Nz, F, J.upd, &J.pml, K, J.dt, sc, J.tz, bx, by, bz);
+    flux_tile<S, float, STAGE, kBatchRows, true>(L, J.Nx, J.Ny, J.Nz, F, J.upd, &J.pml, K, J.dt, sc, J.tz, bx, by,
+                                                 bz);
 }
 
 template <typename S> struct PatchJob {

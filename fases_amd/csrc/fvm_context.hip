@@ -469,8 +469,25 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
         if ((rc = dalloc(ctx, &ctx->pml.psi_vy, ny_t))) return rc;
         if ((rc = dalloc(ctx, &ctx->pml.psi_pz, nz_t))) return rc;
         if ((rc = dalloc(ctx, &ctx->pml.psi_wz, nz_t))) return rc;
-        // device copy for the tiled kernels, which read it only in their cPML branch
+        // device copy for the cPML kernels
         if ((rc = upload(ctx, &ctx->d_pml, &ctx->pml, 1))) return rc;
+        // packed list of the updated cells inside any band (raster order), for the tiled
+        // path's separate cPML pass
+        std::vector<int> bandc;
+        if (ctx->pml.on) {
+            const auto &bxa = ctx->h_pml_band[0], &bya = ctx->h_pml_band[1], &bza = ctx->h_pml_band[2];
+            for (int k = 0; k < Nz; ++k)
+                for (int j = 0; j < Ny; ++j)
+                    for (int i = 0; i < Nx; ++i) {
+                        const long long C = i + (long long)Nx * (j + (long long)Ny * k);
+                        if (ctx->h_upd[(size_t)C] == 1 && (bxa[(size_t)i] >= 0 || bya[(size_t)j] >= 0 || bza[(size_t)k] >= 0))
+                            bandc.push_back((int)C);
+                    }
+        }
+        ctx->n_pml_cells = (int)bandc.size();
+        ctx->d_pml_cells = nullptr;
+        if (!bandc.empty() && (rc = upload(ctx, &ctx->d_pml_cells, bandc.data(), bandc.size()))) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));   // bandc is a local
     }
     ctx->n_src = 0; ctx->n_rec = 0; ctx->ir_cap = 0; ctx->step = 0;
     ctx->tz = 0;

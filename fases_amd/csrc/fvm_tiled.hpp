@@ -229,9 +229,86 @@ template <typename S> struct TileFields {
     S *op, *ou, *ov, *ow;
 };
 
+// cPML factors of a tile's own column (x) and row (y), constant along the z march: band
+// index (-1 outside), 1/kappa - 1, b and a of the psi update (pml_apply's tables).
+template <typename R> struct PmlLane {
+    int bx, by;
+    R kx, bbx, ax, ky, bby, ay;
+};
+template <typename R>
+__device__ inline PmlLane<R> pml_lane(const PmlSet &pm, int ic, int jc) {
+    PmlLane<R> L{-1, -1, R(0), R(0), R(0), R(0), R(0), R(0)};
+    L.bx = pm.x.band[ic];
+    if (L.bx >= 0) { L.kx = (R)pm.x.kc[ic]; L.bbx = (R)pm.x.b[ic]; L.ax = (R)pm.x.a[ic]; }
+    L.by = pm.y.band[jc];
+    if (L.by >= 0) { L.ky = (R)pm.y.kc[jc]; L.bby = (R)pm.y.b[jc]; L.ay = (R)pm.y.a[jc]; }
+    return L;
+}
+// psi of the lane's x band (px, ux) and the wave's y band (py, vy) at plane k, in compute
+// precision (loaded a plane ahead by the tile).
+template <typename R> struct PmlPsi {
+    R px, ux, py, vy;
+};
+template <typename R>
+__device__ inline PmlPsi<R> pml_psi_xy(const PmlSet &pml, const PmlLane<R> &PL, int i, int j, int k, int Nx, int Ny) {
+    PmlPsi<R> q{R(0), R(0), R(0), R(0)};
+    if (PL.bx >= 0) {
+        const long long X = PL.bx + (long long)(2 * pml.d) * (j + (long long)Ny * k);
+        q.px = (R)pml.psi_px[X];
+        q.ux = (R)pml.psi_ux[X];
+    }
+    if (PL.by >= 0) {
+        const long long Y = i + (long long)Nx * (PL.by + (long long)(2 * pml.d) * k);
+        q.py = (R)pml.psi_py[Y];
+        q.vy = (R)pml.psi_vy[Y];
+    }
+    return q;
+}
+// pml_apply with the x and y factors taken from PmlLane and the x, y psi from PmlPsi
+// instead of the tables: the same operations in the same order, so the same bits.
+template <typename R, bool ADVANCE>
+__device__ inline void pml_tile(const PmlSet &pml, const PmlLane<R> &PL, const PmlPsi<R> &Q, bool z_band, int i, int j,
+                                int k, int Nx, int Ny, R &po, R &uo, R &vo, R &wo, R Lpx, R Lpy, R Lpz, R Lux, R Lvy,
+                                R Lwz, R wdt) {
+#pragma clang fp contract(off)
+    if (PL.bx >= 0) {
+        const long long X = PL.bx + (long long)(2 * pml.d) * (j + (long long)Ny * k);
+        const R px = Q.px, ux = Q.ux;
+        po = po + wdt * (PL.kx * Lpx + px);
+        uo = uo + wdt * (PL.kx * Lux + ux);
+        if (ADVANCE) {
+            pml.psi_px[X] = (double)(PL.bbx * px + PL.ax * Lpx);
+            pml.psi_ux[X] = (double)(PL.bbx * ux + PL.ax * Lux);
+        }
+    }
+    if (PL.by >= 0) {
+        const long long Y = i + (long long)Nx * (PL.by + (long long)(2 * pml.d) * k);
+        const R py = Q.py, vy = Q.vy;
+        po = po + wdt * (PL.ky * Lpy + py);
+        vo = vo + wdt * (PL.ky * Lvy + vy);
+        if (ADVANCE) {
+            pml.psi_py[Y] = (double)(PL.bby * py + PL.ay * Lpy);
+            pml.psi_vy[Y] = (double)(PL.bby * vy + PL.ay * Lvy);
+        }
+    }
+    const int bz = z_band ? pml.z.band[k] : -1;    // planes clear of the z bands skip the lookup
+    if (bz >= 0) {
+        const long long Z = i + (long long)Nx * (j + (long long)Ny * bz);
+        const R kc = (R)pml.z.kc[k];
+        const R pz = (R)pml.psi_pz[Z], wz = (R)pml.psi_wz[Z];
+        po = po + wdt * (kc * Lpz + pz);
+        wo = wo + wdt * (kc * Lwz + wz);
+        if (ADVANCE) {
+            const R b = (R)pml.z.b[k], a = (R)pml.z.a[k];
+            pml.psi_pz[Z] = (double)(b * pz + a * Lpz);
+            pml.psi_wz[Z] = (double)(b * wz + a * Lwz);
+        }
+    }
+}
+
 // One tile = 64 x NW threads; outputs 62 columns x NW rows x tz planes of tile
 // (bx, by, bz) of an Nx x Ny x Nz grid.
-template <typename S, typename R, int STAGE, int NW>
+template <typename S, typename R, int STAGE, int NW, bool PML_FUSED>
 __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, const TileFields<S> &F,
                                  const uint8_t *__restrict__ upd, const PmlSet *__restrict__ pmlp, const TileK<R> &K, R dt,
                                  const Scl<R> &sc, int tz, int bx, int by, int bz) {
@@ -360,15 +437,25 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     // The tables stay in memory (read only in that branch), which keeps the plane loop's
     // scalar registers for the field planes.
     int pz_lo = INT_MIN, pz_hi = INT_MAX;           // no plane
-    {
+    int zc0 = INT_MIN, zc1 = INT_MAX;               // planes [zc0, zc1) lie in no z band
+    PmlLane<R> PL{-1, -1, R(0), R(0), R(0), R(0), R(0), R(0)};
+    if constexpr (PML_FUSED) {
         const PmlSet &pm = *pmlp;
         if (pm.on) {
             const bool pml_xy = x0 < pm.clear0[0] || x0 + TXO > pm.clear1[0] || j0 < pm.clear0[1] ||
                                 j0 + NW > pm.clear1[1];
             pz_lo = pml_xy ? INT_MAX : pm.clear0[2];    // every plane / the z bands
             pz_hi = pml_xy ? INT_MIN : pm.clear1[2];
+            zc0 = pm.clear0[2];
+            zc1 = pm.clear1[2];
+            PL = pml_lane<R>(pm, ic, jc);
         }
     }
+    // tiles reaching an x or y band prefetch the lane's psi of the next plane (their cells
+    // otherwise wait on the psi loads inside the plane)
+    const bool pml_xy_tile = PML_FUSED && pz_lo == INT_MAX;
+    PmlPsi<R> psi_n{R(0), R(0), R(0), R(0)};
+    if (pml_xy_tile && k0 < kend) psi_n = pml_psi_xy<R>(*pmlp, PL, ic, jc, k0, Nx, Ny);
 
     // One plane k.
     auto plane = [&](int k) __device__ __attribute__((always_inline)) {
@@ -386,6 +473,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         const R uc = CV::ld(nu_s, sc.s[1]), vc = CV::ld(nv_s, sc.s[2]);
         const S b0 = nb0, b1 = nb1, b2 = nb2, b3 = nb3;
         const unsigned flag = nflag;
+        const PmlPsi<R> psi_c = psi_n;
         R hv[NH];
 #pragma unroll
         for (int q = 0; q < NH; ++q) hv[q] = h_on[q] ? CV::ld(nh_s[q], hsc[q]) : R(0);
@@ -408,6 +496,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
             const long long p4 = nxy * (long long)(k + 4);
             pend_p = LDR(ip, p4);
             pend_w = LDR(iw, p4);
+            if (pml_xy_tile) psi_n = pml_psi_xy<R>(*pmlp, PL, ic, jc, k + 1, Nx, Ny);
         };
         if constexpr (kRaw) issue_next();
         // ---- fill the plane tiles
@@ -495,8 +584,9 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
                     wgt = R(2) / R(3);
                 }
             }
-            if (k < pz_lo || k >= pz_hi)
-                pml_apply<R, STAGE == 2>(*pmlp, i, j, k, Nx, Ny, po, uo, vo, wo, (-FxP_p + FxM_p) * K.ih,
+            if (PML_FUSED && (k < pz_lo || k >= pz_hi))
+                pml_tile<R, STAGE == 2>(*pmlp, PL, psi_c, k < zc0 || k >= zc1, i, j, k, Nx, Ny, po, uo, vo, wo,
+                                        (-FxP_p + FxM_p) * K.ih,
                                          (-FyP_p + FyM_p) * K.ih, (-FzP_p + FzP) * K.ih, Du * K.ih, Dv * K.ih,
                                          Dw * K.ih, wgt * dt);
             ST(ou, pl, uo, sc.is[1]);
@@ -549,7 +639,63 @@ k_flux_tiled(int Nx, int Ny, int Nz, TileFields<S> F, const uint8_t *__restrict_
     const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
     const int bx = (int)(t % (unsigned)nbx), by = (int)((t / (unsigned)nbx) % (unsigned)nby);
     const int bz = (int)(t / ((unsigned)nbx * (unsigned)nby));
-    flux_tile<S, R, STAGE, NW>(L, Nx, Ny, Nz, F, upd, pml, K, dt, sc, tz, bx, by, bz);
+    flux_tile<S, R, STAGE, NW, false>(L, Nx, Ny, Nz, F, upd, pml, K, dt, sc, tz, bx, by, bz);
+}
+
+// cPML of one stage on the packed list of band cells, after k_flux_tiled (which leaves
+// the band cells at their plain RK update): each cell's flux differences along the axes
+// whose band holds it are formed again with the tile's own reconstructions and fluxes
+// (weno_pair / weno_left_of / weno_right_of, godunov_f on the stage input: the same values
+// bit for bit), then pml_apply adds the stretching terms to the stored stage output and,
+// on the last stage, advances psi (functions3D.cuh:681-756). Cells of an FP64 source patch
+// (update flag 2) are left to the patch kernel. Out of the flux kernel, the band code no
+// longer holds registers in every tile of the room.
+template <typename S, typename R, int STAGE>
+__global__ void __launch_bounds__(256)
+k_pml_band(int Nx, int Ny, TileFields<S> F, const uint8_t *__restrict__ upd, const int *__restrict__ cells, int n,
+           const PmlSet *__restrict__ pmlp, TileK<R> K, R dt, Scl<R> sc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long long C = cells[t];
+    if (upd[C] != 1) return;
+    typedef Conv<S, R> CV;
+    const PmlSet &pm = *pmlp;
+    const long long nxy = (long long)Nx * Ny;
+    const int i = (int)(C % Nx), j = (int)((C / Nx) % Ny), k = (int)(C / nxy);
+    // (-F_{+1/2} + F_{-1/2}) / h of p and of the axis velocity f along stride st
+    auto diffs = [&](long long st, const S *f, R sf, R &Lp, R &Lf) {
+        R p[7], q[7];
+#pragma unroll
+        for (int m = 0; m < 7; ++m) {
+            p[m] = CV::ld(F.ip[C + (m - 3) * st], sc.s[0]);
+            q[m] = CV::ld(f[C + (m - 3) * st], sf);
+        }
+        R Ap, Bp, Aq, Bq;
+        const R Amp = weno_left_of(p[0], p[1], p[2], p[3], p[4]);      // cell - 1: left state
+        const R Amq = weno_left_of(q[0], q[1], q[2], q[3], q[4]);
+        weno_pair(p[1], p[2], p[3], p[4], p[5], Ap, Bp);               // the cell: both
+        weno_pair(q[1], q[2], q[3], q[4], q[5], Aq, Bq);
+        const R Bnp = weno_right_of(p[2], p[3], p[4], p[5], p[6]);     // cell + 1: right state
+        const R Bnq = weno_right_of(q[2], q[3], q[4], q[5], q[6]);
+        R FPp, FPq, FMp, FMq;
+        godunov_f(Ap, Bnp, Aq, Bnq, K, FPp, FPq);                      // face + 1/2
+        godunov_f(Amp, Bp, Amq, Bq, K, FMp, FMq);                      // face - 1/2
+        Lp = (-FPp + FMp) * K.ih;
+        Lf = (-FPq + FMq) * K.ih;
+    };
+    R Lpx = R(0), Lux = R(0), Lpy = R(0), Lvy = R(0), Lpz = R(0), Lwz = R(0);
+    const bool bx = pm.x.band[i] >= 0, by = pm.y.band[j] >= 0, bz = pm.z.band[k] >= 0;
+    if (bx) diffs(1, F.iu, sc.s[1], Lpx, Lux);
+    if (by) diffs(Nx, F.iv, sc.s[2], Lpy, Lvy);
+    if (bz) diffs(nxy, F.iw, sc.s[3], Lpz, Lwz);
+    R po = CV::ld(F.op[C], sc.s[0]), uo = CV::ld(F.ou[C], sc.s[1]);
+    R vo = CV::ld(F.ov[C], sc.s[2]), wo = CV::ld(F.ow[C], sc.s[3]);
+    const R wgt = STAGE == 0 ? R(1) : (STAGE == 1 ? R(0.25) : R(2) / R(3));
+    pml_apply<R, STAGE == 2>(pm, i, j, k, Nx, Ny, po, uo, vo, wo, Lpx, Lpy, Lpz, Lux, Lvy, Lwz, wgt * dt);
+    F.op[C] = CV::st(po, sc.is[0]);
+    if (bx) F.ou[C] = CV::st(uo, sc.is[1]);
+    if (by) F.ov[C] = CV::st(vo, sc.is[2]);
+    if (bz) F.ow[C] = CV::st(wo, sc.is[3]);
 }
 
 }  // namespace fasesdev
