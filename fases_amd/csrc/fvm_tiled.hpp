@@ -402,25 +402,18 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         zp.init(p7);
         zw.init(w7);
     }
-    // values in flight, in storage precision: own u, v, update flag, base set (stages 1,
-    // 2), halo items, and p, w of plane k+3 for the z window. Each plane converts what
+    // values in flight, in storage precision: own u, v, update flag, halo items, and p, w
+    // of plane k+3 for the z window. Each plane converts what
     // the previous plane loaded and at once issues the loads of the next one, so a whole
     // plane of work covers their HBM latency.
     constexpr bool kRaw = !std::is_same<S, R>::value;      // storage needs a conversion
     S nu_s, nv_s, nh_s[NH], pend_p, pend_w;
-    S nb0 = S(), nb1 = S(), nb2 = S(), nb3 = S();
     unsigned nflag;
     {
         const long long pl = nxy * (long long)k0;
         nu_s = LDR(iu, pl);
         nv_s = LDR(iv, pl);
         nflag = plane_ld_u8(upd + pl, (unsigned)nxy, col);
-        if (STAGE != 0) {
-            nb0 = LDR(bp, pl);
-            nb1 = LDR(bu, pl);
-            nb2 = LDR(bv, pl);
-            nb3 = LDR(bw, pl);
-        }
 #pragma unroll
         for (int q = 0; q < NH; ++q)                       // per-lane field: no uniform base
             nh_s[q] = h_on[q] ? (hsrc[q] + pl)[h_off[q]] : S();
@@ -471,7 +464,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         }
         const R pc = zp.c0, wc = zw.c0;
         const R uc = CV::ld(nu_s, sc.s[1]), vc = CV::ld(nv_s, sc.s[2]);
-        const S b0 = nb0, b1 = nb1, b2 = nb2, b3 = nb3;
+
         const unsigned flag = nflag;
         const PmlPsi<R> psi_c = psi_n;
         R hv[NH];
@@ -484,12 +477,6 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
             nu_s = LDR(iu, pn);
             nv_s = LDR(iv, pn);
             nflag = plane_ld_u8(upd + pn, (unsigned)nxy, col);
-            if (STAGE != 0) {
-                nb0 = LDR(bp, pn);
-                nb1 = LDR(bu, pn);
-                nb2 = LDR(bv, pn);
-                nb3 = LDR(bw, pn);
-            }
 #pragma unroll
             for (int q = 0; q < NH; ++q)
                 if (h_on[q]) nh_s[q] = (hsrc[q] + pn)[h_off[q]];
@@ -508,6 +495,13 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
             if (h_on[q]) *hdst[q] = hv[q];
         __syncthreads();
         if constexpr (!kRaw) issue_next();
+        S b0 = S(), b1 = S(), b2 = S(), b3 = S();        // RK base set of this plane
+        if (STAGE != 0) {
+            b0 = LDR(bp, pl);
+            b1 = LDR(bu, pl);
+            b2 = LDR(bv, pl);
+            b3 = LDR(bw, pl);
+        }
         // ---- y states of the own row, and of the halo rows on four waves
         {
             R A, B, Av, Bv;
@@ -630,8 +624,13 @@ __device__ inline unsigned xcd_tile(unsigned w, unsigned n) {
     return x * q + min(x, r) + loc;
 }
 
+// The 8-row tile is held to 80 VGPRs (two spilled) so that three blocks share a CU and
+// cover each other's barriers: 717 us per launch (mean of the three stages) against 770
+// with two blocks per CU at 84-96 VGPRs. Stages 1 and 2 fit because their RK base set is
+// loaded in its own plane, after the first barrier, rather than a plane ahead (ahead, the
+// four values stay live across the whole plane: 19-27 spills at 80 VGPRs, 905 us).
 template <typename S, typename R, int STAGE, int NW>
-__global__ void __launch_bounds__(64 * NW)
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 6 : 1, 8)))
 k_flux_tiled(int Nx, int Ny, int Nz, TileFields<S> F, const uint8_t *__restrict__ upd, const PmlSet *__restrict__ pml,
              TileK<R> K, R dt,
              Scl<R> sc, int tz, int nbx, int nby) {
