@@ -41,8 +41,10 @@ template <typename S> struct FluxJob {
 // each other's barriers (the faster shape on the 512x384x256 room and on the batch rooms).
 constexpr int kBatchRows = 8;
 
-template <typename S, int STAGE>
-__global__ void __launch_bounds__(64 * kBatchRows)
+// PF: the cPML fused in the tile (some job of the phase has a band). Without it the tile
+// is held to 80 VGPRs, three blocks per CU, like the single-context kernel.
+template <typename S, int STAGE, bool PF>
+__global__ void __launch_bounds__(64 * kBatchRows) __attribute__((amdgpu_waves_per_eu(PF ? 1 : 6, 8)))
 k_flux_batch(const FluxJob<S> *__restrict__ jobs, const int *__restrict__ boff, int nj, int parity) {
     __shared__ TileLds<float, kBatchRows> L;
     const int j = job_of(boff, nj, blockIdx.x);
@@ -55,8 +57,7 @@ k_flux_batch(const FluxJob<S> *__restrict__ jobs, const int *__restrict__ boff, 
                           J.U[pb][2], J.U[pb][3], J.U[po][0], J.U[po][1], J.U[po][2], J.U[po][3]};
     const TileK<float> K = J.K;
     const Scl<float> sc = J.sc;
-    flux_tile<S, float, STAGE, kBatchRows, true>(L, J.Nx, J.Ny, J.Nz, F, J.upd, &J.pml, K, J.dt, sc, J.tz, bx, by,
-                                                 bz);
+    flux_tile<S, float, STAGE, kBatchRows, PF>(L, J.Nx, J.Ny, J.Nz, F, J.upd, &J.pml, K, J.dt, sc, J.tz, bx, by, bz);
 }
 
 template <typename S> struct PatchJob {
@@ -187,6 +188,7 @@ __global__ void k_step_advance(long long *d_n) {
 // Device tables of one phase (the jobs still running).
 template <typename S> struct Phase {
     int nj = 0, nblk = 0, npatch = 0, nsrc = 0, nbnd = 0, nrec = 0;
+    bool any_pml = false;                // some job has a cPML band: the fused tile
     FluxJob<S> *flux = nullptr;
     PatchJob<S> *patch = nullptr;
     SrcJob<S> *src = nullptr;
@@ -248,6 +250,7 @@ int build_phase(Batch &bt, const std::vector<fases_ctx *> &jobs, const std::vect
         for (int s = 0; s < 3; ++s)
             for (int v = 0; v < 4; ++v) f.U[s][v] = (S *)c->U[s][v];
         f.upd = c->d_upd; f.pml = c->pml; f.K = tile_constants<float>(P); f.dt = (float)P.dt; f.sc = scf;
+        ph.any_pml = ph.any_pml || c->pml.on;
         fj.push_back(f);
         boff.push_back(boff.back() + f.nbx * f.nby * nbz);
         if (c->n_patch > 0) {
@@ -311,12 +314,16 @@ void launch_batch_step(hipStream_t st, const Phase<S> &ph, int np_jobs, int ns_j
                        int parity, long long *d_n) {
     for (int s = 0; s < 3; ++s) {
         if (ph.nblk > 0) {
-            if (s == 0)
-                hipLaunchKernelGGL((k_flux_batch<S, 0>), dim3(ph.nblk), dim3(64, kBatchRows), 0, st, ph.flux, ph.boff, ph.nj, parity);
-            else if (s == 1)
-                hipLaunchKernelGGL((k_flux_batch<S, 1>), dim3(ph.nblk), dim3(64, kBatchRows), 0, st, ph.flux, ph.boff, ph.nj, parity);
-            else
-                hipLaunchKernelGGL((k_flux_batch<S, 2>), dim3(ph.nblk), dim3(64, kBatchRows), 0, st, ph.flux, ph.boff, ph.nj, parity);
+            const dim3 g(ph.nblk), b(64, kBatchRows);
+            if (ph.any_pml) {
+                if (s == 0) hipLaunchKernelGGL((k_flux_batch<S, 0, true>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
+                else if (s == 1) hipLaunchKernelGGL((k_flux_batch<S, 1, true>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
+                else hipLaunchKernelGGL((k_flux_batch<S, 2, true>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
+            } else {
+                if (s == 0) hipLaunchKernelGGL((k_flux_batch<S, 0, false>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
+                else if (s == 1) hipLaunchKernelGGL((k_flux_batch<S, 1, false>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
+                else hipLaunchKernelGGL((k_flux_batch<S, 2, false>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
+            }
         }
         if (ph.npatch > 0) {
             const dim3 g((ph.npatch + 63) / 64);
