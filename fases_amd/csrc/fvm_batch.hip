@@ -136,6 +136,8 @@ template <typename S> struct BndJob {
     double *phi[3], *psi1[3], *psi2[3];
     double rho, c, dt;
     Scl<double> sc;
+    AdeRk rk;                 // stage-collapsed ADE (reduced-precision storage)
+    double *aux;
 };
 
 template <typename S>
@@ -155,9 +157,22 @@ k_boundary_batch(const BndJob<S> *__restrict__ jobs, const int *__restrict__ off
     const int pl = phys(l, parity), pm = phys(1, parity), pr = phys(r, parity);
     const FieldSet<S> fi{J.U[pi][0], J.U[pi][1], J.U[pi][2], J.U[pi][3]};
     const FieldSet<S> fo{J.U[po][0], J.U[po][1], J.U[po][2], J.U[po][3]};
-    const AdeBufs ab{J.phi[pl], J.phi[pm], J.phi[pr], J.psi1[pl], J.psi1[pm], J.psi1[pr],
-                     J.psi2[pl], J.psi2[pm], J.psi2[pr]};
-    boundary_cell<S, double>(t - off[j], J.g, fi, fo, J.bd, J.ade, J.rho, J.c, J.dt, a, b, cc, ab, J.sc);
+    if constexpr (std::is_same<S, double>::value) {
+        const AdeBufs ab{J.phi[pl], J.phi[pm], J.phi[pr], J.psi1[pl], J.psi1[pm], J.psi1[pr],
+                         J.psi2[pl], J.psi2[pm], J.psi2[pr]};
+        boundary_cell<S, double>(t - off[j], J.g, fi, fo, J.bd, J.ade, J.rho, J.c, J.dt, a, b, cc, ab, J.sc);
+    } else {
+        // the single-context loop's stage-collapsed ADE: x0 = phi[mid], x3 -> phi[logical 0]
+        const int p0 = phys(0, parity);
+        const AdeRkBufs rb{J.phi[pm], J.psi1[pm], J.psi2[pm], J.phi[p0], J.psi1[p0], J.psi2[p0], J.aux};
+        const double Z0 = J.rho * J.c, iz = 1.0 / (2.0 * Z0);
+        if (stage == 0)
+            boundary_cell_rk<S, double, 0>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
+        else if (stage == 1)
+            boundary_cell_rk<S, double, 1>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
+        else
+            boundary_cell_rk<S, double, 2>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
+    }
 }
 
 template <typename S> struct RecJob {
@@ -284,6 +299,8 @@ int build_phase(Batch &bt, const std::vector<fases_ctx *> &jobs, const std::vect
                         c->d_Keff, c->d_Leff};
             for (int k = 0; k < 3; ++k) { b.phi[k] = c->phi[k]; b.psi1[k] = c->psi1[k]; b.psi2[k] = c->psi2[k]; }
             b.rho = P.rho; b.c = P.c; b.dt = P.dt; b.sc = scd;
+            b.rk = ade_rk_of(c);
+            b.aux = c->d_ade_aux;
             bj.push_back(b);
             doff.push_back(doff.back() + c->nB);
         }

@@ -529,6 +529,13 @@ int fases_set_materials(fases_ctx *ctx, int nMat, int K, int L, const double *Ri
         if ((rc = dalloc_replace(ctx, &ctx->psi1[q], (size_t)ctx->nB * L))) return rc;
         if ((rc = dalloc_replace(ctx, &ctx->psi2[q], (size_t)ctx->nB * L))) return rc;
     }
+    if (ctx->prm.precision != FASES_FP64) {       // stage-collapsed ADE (boundary_cell_rk)
+        const std::vector<double> rk = ade_rk_table(nMat, K, L, Rinf, A, zeta, B, C, alpha, beta, ctx->prm.dt);
+        ctx->ade_nc = 6 * K + 14 * L + 4;
+        if ((rc = upload_replace(ctx, &ctx->d_ade_rk, rk.data(), rk.size()))) return rc;
+        if ((rc = dalloc_replace(ctx, &ctx->d_ade_aux, (size_t)ctx->nB * 3))) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));   // rk is a local
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_mat = true;
     return FASES_OK;

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace fasesdev;
@@ -58,6 +59,10 @@ struct fases_ctx {
            *d_alpha = nullptr, *d_beta = nullptr;
     int *d_Keff = nullptr, *d_Leff = nullptr;
     double *phi[3] = {}, *psi1[3] = {}, *psi2[3] = {};   // [0]=phi, [1]=phi_old, [2]=phi_nu; [K or L][nB]
+    // stage-collapsed ADE of reduced-precision storage: per-material coefficients
+    // (boundary_cell_rk's layout, ade_nc doubles each) and the [3][nB] stage values T1, w0, w1
+    double *d_ade_rk = nullptr, *d_ade_aux = nullptr;
+    int ade_nc = 0;
     // PML (device tables in pml, host copies per axis x, y, z)
     PmlSet pml{};
     PmlSet *d_pml = nullptr;      // device copy of pml (tiled path's cPML pass)
@@ -168,6 +173,66 @@ int upload_replace(fases_ctx *ctx, T **slot, const T *src, size_t count) {
     *slot = nullptr;
     if (rc) return rc;
     return upload(ctx, slot, src, count);
+}
+
+// Coefficients of the stage-collapsed ADE (boundary_cell_rk) for nMat materials, in FP64:
+// per real pole a = 1 - dt zeta, per complex pair M = [[m, -n], [n, m]] with
+// m = 1 - dt alpha, n = dt beta (the step x' = x + dt (w - zeta x), resp.
+// psi1' = psi1 + dt (w - alpha psi1 - beta psi2), psi2' = psi2 + dt (beta psi1 - alpha psi2)).
+inline std::vector<double> ade_rk_table(int nMat, int K, int L, const double *Rinf, const double *A,
+                                        const double *zeta, const double *B, const double *C, const double *alpha,
+                                        const double *beta, double dt) {
+    const int nc = 6 * K + 14 * L + 4;
+    std::vector<double> t((size_t)nMat * nc, 0.0);
+    for (int m = 0; m < nMat; ++m) {
+        double *cf = t.data() + (size_t)m * nc;
+        double sumA = 0.0, sumAa = 0.0, sumB = 0.0, sumBC1 = 0.0;
+        for (int k = 0; k < K; ++k) {
+            const double Ak = A[(size_t)m * K + k], a = 1.0 - dt * zeta[(size_t)m * K + k];
+            double *q = cf + 6 * k;
+            q[0] = Ak;
+            q[1] = Ak * a;
+            q[2] = Ak * (3.0 + a * a) / 4.0;
+            q[3] = 1.0 / 3.0 + a * (3.0 + a * a) / 6.0;
+            q[4] = a * a * dt / 6.0;
+            q[5] = a * dt / 6.0;
+            sumA += Ak;
+            sumAa += Ak * a;
+        }
+        for (int l = 0; l < L; ++l) {
+            const double b_ = B[(size_t)m * L + l], c_ = C[(size_t)m * L + l];
+            const double mm = 1.0 - dt * alpha[(size_t)m * L + l], nn = dt * beta[(size_t)m * L + l];
+            // M, M^2, P2 = (3I + M^2)/4, P3 = I/3 + (2/3) M P2 (row-major 2x2)
+            const double M[4] = {mm, -nn, nn, mm};
+            const double M2[4] = {mm * mm - nn * nn, -2.0 * mm * nn, 2.0 * mm * nn, mm * mm - nn * nn};
+            const double P2[4] = {(3.0 + M2[0]) / 4.0, M2[1] / 4.0, M2[2] / 4.0, (3.0 + M2[3]) / 4.0};
+            double P3[4];
+            for (int r = 0; r < 2; ++r)
+                for (int s = 0; s < 2; ++s)
+                    P3[2 * r + s] = (r == s ? 1.0 / 3.0 : 0.0) +
+                                    (2.0 / 3.0) * (M[2 * r] * P2[s] + M[2 * r + 1] * P2[2 + s]);
+            double *q = cf + 6 * K + 14 * l;
+            q[0] = b_;
+            q[1] = c_;
+            q[2] = b_ * M[0] + c_ * M[2];      // [B C] M
+            q[3] = b_ * M[1] + c_ * M[3];
+            q[4] = b_ * P2[0] + c_ * P2[2];    // [B C] P2
+            q[5] = b_ * P2[1] + c_ * P2[3];
+            for (int e = 0; e < 4; ++e) q[6 + e] = P3[e];
+            q[10] = dt / 6.0 * M2[0];          // dt/6 M^2 e1
+            q[11] = dt / 6.0 * M2[2];
+            q[12] = dt / 6.0 * M[0];           // dt/6 M e1
+            q[13] = dt / 6.0 * M[2];
+            sumB += b_;
+            sumBC1 += b_ * mm + c_ * nn;       // [B C] M e1
+        }
+        double *tl = cf + 6 * K + 14 * L;
+        tl[0] = Rinf[m];
+        tl[1] = Rinf[m] + dt * (sumA + sumB);
+        tl[2] = dt / 4.0 * (sumAa + sumBC1);
+        tl[3] = Rinf[m] + dt / 4.0 * (sumA + sumB);
+    }
+    return t;
 }
 
 int set_error(fases_ctx *ctx, int code, const std::string &msg) {
@@ -306,6 +371,15 @@ int autotune_tile(fases_ctx *ctx) {
     return FASES_OK;
 }
 
+// Stage-collapsed ADE parameters and buffers of a context (reduced-precision storage):
+// x0 = phi[1] (the step's start state), x3 -> phi[0].
+inline AdeRk ade_rk_of(const fases_ctx *ctx) {
+    return AdeRk{ctx->K, ctx->L, ctx->ade_nc, ctx->d_ade_rk, ctx->d_Keff, ctx->d_Leff, 2.0 * ctx->prm.dt / 3.0};
+}
+inline AdeRkBufs ade_rk_bufs(const fases_ctx *ctx) {
+    return AdeRkBufs{ctx->phi[1], ctx->psi1[1], ctx->psi2[1], ctx->phi[0], ctx->psi1[0], ctx->psi2[0], ctx->d_ade_aux};
+}
+
 // The three stages of step n (flux -> source patch -> sources -> boundary each) on
 // explicit field sets: sets[0] = U^n (also the RK base), stage 0 writes sets[1], stage 1
 // sets[2], stage 2 sets[3] (= sets[1] in the plain loop, whose caller then rotates);
@@ -381,13 +455,30 @@ int launch_stages(fases_ctx *ctx, long long n, const FieldSet<S> sets[4], double
                                sets[q.out].p, ctx->d_scell, ctx->d_sval, ctx->d_sfreq, ctx->d_us,
                                ctx->src_opt == FASES_SRC_SINE ? ctx->n_samp : 0LL, ctx->n_src, ctx->src_opt, q.nn, q.w, P.dt, P.c, P.rho, As, P.h,
                                ctx->n_patch > 0 ? ctx->d_pidx : nullptr, sh[q.out], sc.s[0], sc.is[0]);
-        if (ctx->nB > 0)
+        if (ctx->nB > 0 && std::is_same<S, double>::value)
             hipLaunchKernelGGL((k_boundary<S, G>), dim3((ctx->nB + sb - 1) / sb), dim3(sb), 0, ctx->stream, g,
                                sets[q.in], sets[q.out], bd, ade, P.rho, P.c, P.dt, q.a, q.b, q.c,
                                AdeBufs{ctx->phi[q.l], ctx->phi[q.mid], ctx->phi[q.r], ctx->psi1[q.l],
                                        ctx->psi1[q.mid], ctx->psi1[q.r], ctx->psi2[q.l], ctx->psi2[q.mid],
                                        ctx->psi2[q.r]},
                                scd);
+        else if (ctx->nB > 0) {
+            // reduced storage: the stage-collapsed ADE on the step's start state phi[1]
+            // (x0), end state into phi[0] (rotated into phi[1] after the step)
+            const AdeRk rk = ade_rk_of(ctx);
+            const AdeRkBufs rb = ade_rk_bufs(ctx);
+            const double Z0 = P.rho * P.c, iz = 1.0 / (2.0 * Z0);
+            const dim3 bg((unsigned)((ctx->nB + sb - 1) / sb));
+            if (s == 0)
+                hipLaunchKernelGGL((k_boundary_rk<S, G, 0>), bg, dim3(sb), 0, ctx->stream, g, sets[q.in],
+                                   sets[q.out], bd, rk, Z0, iz, rb, scd);
+            else if (s == 1)
+                hipLaunchKernelGGL((k_boundary_rk<S, G, 1>), bg, dim3(sb), 0, ctx->stream, g, sets[q.in],
+                                   sets[q.out], bd, rk, Z0, iz, rb, scd);
+            else
+                hipLaunchKernelGGL((k_boundary_rk<S, G, 2>), bg, dim3(sb), 0, ctx->stream, g, sets[q.in],
+                                   sets[q.out], bd, rk, Z0, iz, rb, scd);
+        }
     }
     return FASES_OK;
 }

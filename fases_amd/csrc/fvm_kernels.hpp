@@ -385,6 +385,139 @@ k_boundary(Grid g, FieldSet<S> in, FieldSet<S> out, Bnd<G> bd, Ade ade, double r
     boundary_cell<S, G>(b, g, in, out, bd, ade, rho, c, dt, as, bs, cs, ab, sc);
 }
 
+// ------------------------------------------------------------------------------------
+// The same ADE recursion with the three SSP-RK3 stages of a step collapsed onto the
+// step's start state x0 (reduced-precision storage; FP64 storage keeps boundary_cell,
+// which is the reference's operation order). Per real pole, with a = 1 - dt zeta and
+// w_s the stage's characteristic wout:
+//   x1 = a x0 + dt w0
+//   x2 = (3 x0 + a x1 + dt w1) / 4 = (3 + a^2)/4 x0 + a dt/4 w0 + dt/4 w1
+//   x3 = (x0 + 2 (a x2 + dt w2)) / 3 = (1/3 + a (3 + a^2)/6) x0 + a^2 dt/6 w0 + a dt/6 w1 + 2 dt/3 w2
+// and per complex pair Y = (psi1, psi2) the same with a -> M = [[1 - dt alpha, -dt beta],
+// [dt beta, 1 - dt alpha]] acting on Y and dt w -> dt w e1. The stage's admittance sum
+// (sum_k A_k x_{s+1,k} + sum_l B_l psi1 + C_l psi2) is then a dot product with x0 plus
+// material constants times w0 .. ws, so the intermediate states x1, x2 never exist:
+//   stage 0: reads x0, keeps T1 = (stage-1 x0 part) + G10 w0 and w0;
+//   stage 1: reads T1, keeps w1;
+//   stage 2: reads x0, w0, w1, writes x3.
+// Per boundary cell and step that is 30 FP64 values moved instead of 64 (K = 4, L = 2).
+// Every constant is formed on the host in FP64 (fases_set_materials).
+// ------------------------------------------------------------------------------------
+// Per-material coefficients, material m at coef + m * nc, nc = 6K + 14L + 4:
+//   real pole k at 6k:        A, A a, A (3 + a^2)/4, 1/3 + a (3 + a^2)/6, a^2 dt/6, a dt/6
+//   pair l at 6K + 14l:       B, C, [B C] M (2), [B C] (3I + M^2)/4 (2),
+//                             P3 = I/3 + M (3I + M^2)/6 (4, row-major), dt/6 M^2 e1 (2), dt/6 M e1 (2)
+//   tail at 6K + 14L:         Rinf, Rinf + G00, G10, Rinf + G11
+enum AdeRkTail { kRkRinf = 0, kRkG00, kRkG10, kRkG11 };
+struct AdeRk {
+    int K, L, nc;
+    const double *coef;
+    const int *Keff, *Leff;
+    double two_dt3;           // 2 dt / 3
+};
+// x0: the step's start state (read), x3: the step's end state (written at stage 2),
+// aux: [3][nB] = T1, w0, w1 (entry-major).
+struct AdeRkBufs {
+    const double *x0_phi, *x0_p1, *x0_p2;
+    double *x3_phi, *x3_p1, *x3_p2;
+    double *aux;
+};
+
+template <typename S, typename G, int STAGE>
+__device__ inline void boundary_cell_rk(int b, const Grid &g, const FieldSet<S> &in, const FieldSet<S> &out,
+                                        const Bnd<G> &bd, const AdeRk &ade, double Z0, double inv_2z0,
+                                        const AdeRkBufs &ab, const Scl<double> &sc) {
+    typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type R;
+    typedef Conv<S, R> CV;
+    const int C = bd.cell[b], base = bd.base[b], mw = bd.mat[b];
+    const int m = mw & 0xFF;
+    const double nx = bd.at(kGeoNx, b), ny = bd.at(kGeoNy, b), nz = bd.at(kGeoNz, b), chi = bd.at(kGeoChi, b);
+    double pD, uD, vD, wD;
+    if (mw & 0x100) {
+        pD = (double)CV::ld(in.p[base], (R)sc.s[0]);
+        uD = (double)CV::ld(in.u[base], (R)sc.s[1]);
+        vD = (double)CV::ld(in.v[base], (R)sc.s[2]);
+        wD = (double)CV::ld(in.w[base], (R)sc.s[3]);
+    } else {
+        const double sx = bd.at(kGeoSx, b), sy = bd.at(kGeoSy, b), sz = bd.at(kGeoSz, b);
+        pD = tri_probe(in.p, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[0]);
+        uD = tri_probe(in.u, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[1]);
+        vD = tri_probe(in.v, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[2]);
+        wD = tri_probe(in.w, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[3]);
+    }
+    const double un = uD * nx + vD * ny + wD * nz;
+    const double utx = uD - un * nx, uty = vD - un * ny, utz = wD - un * nz;
+    const double wout = pD + Z0 * un;
+    const double *__restrict__ cf = ade.coef + (long long)m * ade.nc;
+    const double *__restrict__ tl = cf + 6 * ade.K + 14 * ade.L;
+    const int Ke = ade.Keff[m], Le = ade.Leff[m];
+    const int nB = bd.n;
+    double win;
+    if (STAGE == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int k = 0; k < Ke; ++k) {
+            const double x = ab.x0_phi[ade_at(k, b, nB)];
+            s0 = fma(cf[6 * k + 1], x, s0);
+            s1 = fma(cf[6 * k + 2], x, s1);
+        }
+        for (int l = 0; l < Le; ++l) {
+            const double *q = cf + 6 * ade.K + 14 * l;
+            const long long s = ade_at(l, b, nB);
+            const double y1 = ab.x0_p1[s], y2 = ab.x0_p2[s];
+            s0 = fma(q[2], y1, fma(q[3], y2, s0));
+            s1 = fma(q[4], y1, fma(q[5], y2, s1));
+        }
+        win = fma(tl[kRkG00], wout, s0);
+        ab.aux[b] = fma(tl[kRkG10], wout, s1);
+        ab.aux[(long long)nB + b] = wout;
+    } else if (STAGE == 1) {
+        win = fma(tl[kRkG11], wout, ab.aux[b]);
+        ab.aux[2LL * nB + b] = wout;
+    } else {
+        const double w0 = ab.aux[(long long)nB + b], w1 = ab.aux[2LL * nB + b];
+        const double w2 = ade.two_dt3 * wout;
+        double sum = tl[kRkRinf] * wout;
+        for (int k = 0; k < Ke; ++k) {
+            const long long s = ade_at(k, b, nB);
+            const double *q = cf + 6 * k;
+            const double x3 = fma(q[3], ab.x0_phi[s], fma(q[4], w0, fma(q[5], w1, w2)));
+            ab.x3_phi[s] = x3;
+            sum = fma(q[0], x3, sum);
+        }
+        for (int l = 0; l < Le; ++l) {
+            const double *q = cf + 6 * ade.K + 14 * l;
+            const long long s = ade_at(l, b, nB);
+            const double y1 = ab.x0_p1[s], y2 = ab.x0_p2[s];
+            const double z1 = fma(q[6], y1, fma(q[7], y2, fma(q[10], w0, fma(q[12], w1, w2))));
+            const double z2 = fma(q[8], y1, fma(q[9], y2, fma(q[11], w0, q[13] * w1)));
+            ab.x3_p1[s] = z1;
+            ab.x3_p2[s] = z2;
+            sum = fma(q[0], z1, fma(q[1], z2, sum));
+        }
+        win = sum;
+    }
+    const double pb = 0.5 * (win + wout);
+    const double unb = (wout - win) * inv_2z0;
+    const double ub = unb * nx + utx, vb = unb * ny + uty, wb = unb * nz + utz;
+    const double u_o = (double)CV::ld(out.u[C], (R)sc.s[1]);
+    const double v_o = (double)CV::ld(out.v[C], (R)sc.s[2]);
+    const double w_o = (double)CV::ld(out.w[C], (R)sc.s[3]);
+    const double p_o = (double)CV::ld(out.p[C], (R)sc.s[0]);
+    out.u[C] = CV::st((R)((1.0 - chi) * u_o + chi * ub), (R)sc.is[1]);
+    out.v[C] = CV::st((R)((1.0 - chi) * v_o + chi * vb), (R)sc.is[2]);
+    out.w[C] = CV::st((R)((1.0 - chi) * w_o + chi * wb), (R)sc.is[3]);
+    out.p[C] = CV::st((R)((1.0 - chi) * p_o + chi * pb), (R)sc.is[0]);
+}
+
+template <typename S, typename G, int STAGE>
+__global__ void __launch_bounds__(256)
+k_boundary_rk(Grid g, FieldSet<S> in, FieldSet<S> out, Bnd<G> bd, AdeRk ade, double Z0, double inv_2z0,
+              AdeRkBufs ab, Scl<double> sc) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= bd.n) return;
+    boundary_cell_rk<S, G, STAGE>(b, g, in, out, bd, ade, Z0, inv_2z0, ab, sc);
+}
+
 // Receiver gather after a step: ir[r*stride + n] = p(cell_r), vel[(r*stride + n)*3 + a].
 template <typename S>
 __global__ void k_receivers(FieldSet<S> f, const int *__restrict__ cell, int n_rec, double *__restrict__ ir,
