@@ -218,6 +218,7 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=3000, help="steps of the adjoint benchmark (0: skip)")
     ap.add_argument("--tile-z", type=int, default=0, help="flux kernel z depth per block (0: autotune)")
     ap.add_argument("--tile-rows", type=int, default=0, help="flux kernel waves per block, 8 or 16 (0: autotune)")
+    ap.add_argument("--tile-cols", type=int, default=0, help="flux kernel columns per block, 64 or 62 (0: autotune)")
     ap.add_argument("--no-geometry", action="store_true", help="skip the host vs GPU scene classification timing")
     ap.add_argument("--ga-rays", type=int, default=10000,
                     help="rays per source of the geometrical-acoustics benchmark (0: skip)")
@@ -239,9 +240,10 @@ def main() -> int:
     if acc_steps > total and not args.no_fp64_check:
         # the waveform must cover the accuracy window the run continues into
         wl = W.config(2, room=room, nsteps=acc_steps, precision=args.precision)
-    if args.tile_z or args.tile_rows:
+    if args.tile_z or args.tile_rows or args.tile_cols:
         import dataclasses
-        wl = dataclasses.replace(wl, params=dataclasses.replace(wl.params, tile_z=args.tile_z, tile_rows=args.tile_rows))
+        wl = dataclasses.replace(wl, params=dataclasses.replace(wl.params, tile_z=args.tile_z, tile_rows=args.tile_rows,
+                                                                tile_cols=args.tile_cols))
     sc = wl.scene
     log("scene %s padded %s built in %.1f s" % (room, sc.dims, time.time() - t0))
 

@@ -47,6 +47,7 @@ class fases_params(C.Structure):
         ("adjoint_variant", C.c_int),
         ("tile_rows", C.c_int),
         ("source_patch", C.c_int),
+        ("tile_cols", C.c_int),
     ]
 
 

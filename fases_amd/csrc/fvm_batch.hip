@@ -40,13 +40,16 @@ template <typename S> struct FluxJob {
 // Rows (waves) per block of the batched flux kernel: 8-wave blocks, two per CU, overlap
 // each other's barriers (the faster shape on the 512x384x256 room and on the batch rooms).
 constexpr int kBatchRows = 8;
+// Tile width of the batched flux kernel: 64 columns (the faster width on rooms of up to
+// 256 cells across; results do not depend on the width).
+constexpr int kBatchTX = 64;
 
 // PF: the cPML fused in the tile (some job of the phase has a band). Without it the tile
 // is held to 80 VGPRs, three blocks per CU, like the single-context kernel.
 template <typename S, int STAGE, bool PF>
 __global__ void __launch_bounds__(64 * kBatchRows) __attribute__((amdgpu_waves_per_eu(PF ? 1 : 6, 8)))
 k_flux_batch(const FluxJob<S> *__restrict__ jobs, const int *__restrict__ boff, int nj, int parity) {
-    __shared__ TileLds<float, kBatchRows> L;
+    __shared__ TileLds<float, kBatchRows, kBatchTX> L;
     const int j = job_of(boff, nj, blockIdx.x);
     const FluxJob<S> &J = jobs[j];
     const int lb = blockIdx.x - boff[j];
@@ -57,7 +60,8 @@ k_flux_batch(const FluxJob<S> *__restrict__ jobs, const int *__restrict__ boff, 
                           J.U[pb][2], J.U[pb][3], J.U[po][0], J.U[po][1], J.U[po][2], J.U[po][3]};
     const TileK<float> K = J.K;
     const Scl<float> sc = J.sc;
-    flux_tile<S, float, STAGE, kBatchRows, PF>(L, J.Nx, J.Ny, J.Nz, F, J.upd, &J.pml, K, J.dt, sc, J.tz, bx, by, bz);
+    flux_tile<S, float, STAGE, kBatchRows, kBatchTX, PF>(L, J.Nx, J.Ny, J.Nz, F, J.upd, &J.pml, K, J.dt, sc, J.tz, bx, by,
+                                                         bz);
 }
 
 template <typename S> struct PatchJob {
@@ -259,7 +263,7 @@ int build_phase(Batch &bt, const std::vector<fases_ctx *> &jobs, const std::vect
         const Grid g{c->Nx, c->Ny, c->Nz, (long long)c->Nx * c->Ny};
         FluxJob<S> f{};
         f.Nx = c->Nx; f.Ny = c->Ny; f.Nz = c->Nz; f.tz = c->tz;
-        f.nbx = (c->Nx - 6 + tile::TXO - 1) / tile::TXO;
+        f.nbx = (c->Nx - 6 + kBatchTX - 1) / kBatchTX;
         f.nby = (c->Ny - 6 + kBatchRows - 1) / kBatchRows;
         const int nbz = (c->Nz - 6 + c->tz - 1) / c->tz;
         for (int s = 0; s < 3; ++s)

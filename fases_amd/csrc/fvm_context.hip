@@ -308,6 +308,8 @@ int fases_create(const fases_params *p, fases_ctx **out) {
     if (p->pml_cells < 1) return set_error(nullptr, FASES_E_ARG, "fases_create: pml_cells must be >= 1");
     if (p->tile_rows != 0 && p->tile_rows != 8 && p->tile_rows != 16)
         return set_error(nullptr, FASES_E_ARG, "fases_create: tile_rows must be 0, 8 or 16");
+    if (p->tile_cols != 0 && p->tile_cols != 62 && p->tile_cols != 64)
+        return set_error(nullptr, FASES_E_ARG, "fases_create: tile_cols must be 0, 62 or 64");
     if (p->source_patch < 0 || p->source_patch > 64)
         return set_error(nullptr, FASES_E_ARG, "fases_create: source_patch must be in [0, 64]");
     if (p->tile_z < 0) return set_error(nullptr, FASES_E_ARG, "fases_create: tile_z must be >= 0");
@@ -880,7 +882,7 @@ int fases_pml_tables(const fases_ctx *ctx, int axis, int32_t *band, double *kc, 
 
 int fases_tile_config(const fases_ctx *ctx, int *tile_x, int *tile_y, int *tile_z) {
     if (!ctx) return FASES_E_ARG;
-    if (tile_x) *tile_x = tile::TXO;
+    if (tile_x) *tile_x = ctx->tx;
     if (tile_y) *tile_y = ctx->nw;
     if (tile_z) *tile_z = ctx->tz;
     return FASES_OK;

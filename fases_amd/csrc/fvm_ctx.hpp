@@ -90,6 +90,7 @@ struct fases_ctx {
     unsigned *d_amax = nullptr;
     int tz = 0;                          // z planes per block of the tiled kernel (0: not tuned)
     int nw = 16;                         // waves (rows) per block of the tiled kernel: 16 or 8
+    int tx = 64;                         // columns per block of the tiled kernel: 64 or 62
     void *adj = nullptr;                 // adjoint checkpoints and buffers (fvm_adjoint.hip)
     void (*adj_release)(void *) = nullptr;
 
@@ -274,25 +275,25 @@ TileK<R> tile_constants(const fases_params &P) {
     return K;
 }
 
-template <typename S, typename R, int NW>
+template <typename S, typename R, int NW, int TX>
 int launch_tiled(fases_ctx *ctx, int stage, const FieldSet<S> &fi, const FieldSet<S> &fb, const FieldSet<S> &fo, R rho,
                  R c, R h, R dt, int tz) {
     (void)rho; (void)c; (void)h;
     const TileK<R> K = tile_constants<R>(ctx->prm);
     const dim3 tb(64, NW);
-    const int nbx = (ctx->Nx - 6 + tile::TXO - 1) / tile::TXO, nby = (ctx->Ny - 6 + NW - 1) / NW;
+    const int nbx = (ctx->Nx - 6 + TX - 1) / TX, nby = (ctx->Ny - 6 + NW - 1) / NW;
     const int nbz = (ctx->Nz - 6 + tz - 1) / tz;
     const dim3 tg((unsigned)nbx * (unsigned)nby * (unsigned)nbz);     // 1-D: tiles ordered per XCD in the kernel
     const Scl<R> sc = field_scales<R>(ctx);
     const TileFields<S> F{fi.p, fi.u, fi.v, fi.w, fb.p, fb.u, fb.v, fb.w, fo.p, fo.u, fo.v, fo.w};
     if (stage == 0)
-        hipLaunchKernelGGL((k_flux_tiled<S, R, 0, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
+        hipLaunchKernelGGL((k_flux_tiled<S, R, 0, NW, TX>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
                            ctx->d_upd, ctx->d_pml, K, dt, sc, tz, nbx, nby);
     else if (stage == 1)
-        hipLaunchKernelGGL((k_flux_tiled<S, R, 1, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
+        hipLaunchKernelGGL((k_flux_tiled<S, R, 1, NW, TX>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
                            ctx->d_upd, ctx->d_pml, K, dt, sc, tz, nbx, nby);
     else
-        hipLaunchKernelGGL((k_flux_tiled<S, R, 2, NW>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
+        hipLaunchKernelGGL((k_flux_tiled<S, R, 2, NW, TX>), tg, tb, 0, ctx->stream, ctx->Nx, ctx->Ny, ctx->Nz, F,
                            ctx->d_upd, ctx->d_pml, K, dt, sc, tz, nbx, nby);
     if (ctx->n_pml_cells > 0) {       // cPML of the band cells on the stage output
         const dim3 pg((unsigned)((ctx->n_pml_cells + 255) / 256));
@@ -314,24 +315,28 @@ bool uses_tiled(const fases_ctx *ctx) {
     return !(kv == 1 || (kv == 0 && ctx->prm.precision == FASES_FP64));
 }
 
-// Tiled stage launch with the context's block height (16 waves: one block per CU and
-// less halo; 8 waves: two blocks per CU whose barriers do not wait on each other).
+// Tiled stage launch with a block height (16 waves: one block per CU and less halo; 8
+// waves: three blocks per CU whose barriers do not wait on each other) and width (64
+// output columns with edge work, or 62 without).
 template <typename S, typename R>
 int launch_tiled_nw(fases_ctx *ctx, int stage, const FieldSet<S> &fi, const FieldSet<S> &fb, const FieldSet<S> &fo,
-                    R rho, R c, R h, R dt, int nw, int tz) {
-    return nw == 8 ? launch_tiled<S, R, 8>(ctx, stage, fi, fb, fo, rho, c, h, dt, tz)
-                   : launch_tiled<S, R, 16>(ctx, stage, fi, fb, fo, rho, c, h, dt, tz);
+                    R rho, R c, R h, R dt, int nw, int tz, int tx) {
+    if (tx == 62)
+        return nw == 8 ? launch_tiled<S, R, 8, 62>(ctx, stage, fi, fb, fo, rho, c, h, dt, tz)
+                       : launch_tiled<S, R, 16, 62>(ctx, stage, fi, fb, fo, rho, c, h, dt, tz);
+    return nw == 8 ? launch_tiled<S, R, 8, 64>(ctx, stage, fi, fb, fo, rho, c, h, dt, tz)
+                   : launch_tiled<S, R, 16, 64>(ctx, stage, fi, fb, fo, rho, c, h, dt, tz);
 }
 
 // Tile autotuner: times a stage-0 launch (set 0 -> set 1) plus a stage-1 launch (sets
-// 0, 1 -> set 2) for each candidate block height and z depth, best of four, and keeps
-// the fastest. Sets 1 and 2 are rewritten by the next step before they are read and
-// neither stage advances the PML states, so tuning changes no state. A fixed tile_rows /
-// tile_z restricts the search to that value.
+// 0, 1 -> set 2) for each candidate block width, height and z depth, best of four, and
+// keeps the fastest. Sets 1 and 2 are rewritten by the next step before they are read and
+// neither stage advances the PML states, so tuning changes no state; no candidate changes
+// a result. A fixed tile_cols / tile_rows / tile_z restricts the search to that value.
 template <typename S, typename R>
 int autotune_tile(fases_ctx *ctx) {
-    const int fix_nw = ctx->prm.tile_rows, fix_tz = ctx->prm.tile_z;
-    if (fix_tz > 0 && fix_nw > 0) { ctx->tz = fix_tz; ctx->nw = fix_nw; return FASES_OK; }
+    const int fix_tx = ctx->prm.tile_cols, fix_nw = ctx->prm.tile_rows, fix_tz = ctx->prm.tile_z;
+    if (fix_tz > 0 && fix_nw > 0 && fix_tx > 0) { ctx->tz = fix_tz; ctx->nw = fix_nw; ctx->tx = fix_tx; return FASES_OK; }
     const fases_params &P = ctx->prm;
     const int nz = ctx->Nz - 6;
     const int cand[4] = {16, 32, 64, 128};
@@ -339,28 +344,31 @@ int autotune_tile(fases_ctx *ctx) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     float best = 1e30f;
-    int pick = fix_tz > 0 ? fix_tz : 64, pick_nw = fix_nw > 0 ? fix_nw : 16;
-    for (int nw : {16, 8}) {
-        if (fix_nw > 0 && nw != fix_nw) continue;
-        for (int tz : cand) {
-            if (fix_tz > 0 && tz != fix_tz) continue;
-            if (fix_tz <= 0 && tz > 16 && (tz / 2) >= nz) break;   // deeper blocks than the grid add nothing
-            float tmin = 1e30f;
-            for (int rep = 0; rep < 4; ++rep) {
-                HIPCHK(hipEventRecord(e0, ctx->stream));
-                int rc = launch_tiled_nw<S, R>(ctx, 0, fset<S>(ctx, 0), fset<S>(ctx, 0), fset<S>(ctx, 1), (R)P.rho,
-                                               (R)P.c, (R)P.h, (R)P.dt, nw, tz);
-                if (rc) return rc;
-                rc = launch_tiled_nw<S, R>(ctx, 1, fset<S>(ctx, 1), fset<S>(ctx, 0), fset<S>(ctx, 2), (R)P.rho,
-                                           (R)P.c, (R)P.h, (R)P.dt, nw, tz);
-                if (rc) return rc;
-                HIPCHK(hipEventRecord(e1, ctx->stream));
-                HIPCHK(hipEventSynchronize(e1));
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                tmin = std::min(tmin, ms);
+    int pick = fix_tz > 0 ? fix_tz : 64, pick_nw = fix_nw > 0 ? fix_nw : 16, pick_tx = fix_tx > 0 ? fix_tx : 64;
+    for (int tx : {64, 62}) {
+        if (fix_tx > 0 && tx != fix_tx) continue;
+        for (int nw : {16, 8}) {
+            if (fix_nw > 0 && nw != fix_nw) continue;
+            for (int tz : cand) {
+                if (fix_tz > 0 && tz != fix_tz) continue;
+                if (fix_tz <= 0 && tz > 16 && (tz / 2) >= nz) break;   // deeper blocks than the grid add nothing
+                float tmin = 1e30f;
+                for (int rep = 0; rep < 4; ++rep) {
+                    HIPCHK(hipEventRecord(e0, ctx->stream));
+                    int rc = launch_tiled_nw<S, R>(ctx, 0, fset<S>(ctx, 0), fset<S>(ctx, 0), fset<S>(ctx, 1), (R)P.rho,
+                                                   (R)P.c, (R)P.h, (R)P.dt, nw, tz, tx);
+                    if (rc) return rc;
+                    rc = launch_tiled_nw<S, R>(ctx, 1, fset<S>(ctx, 1), fset<S>(ctx, 0), fset<S>(ctx, 2), (R)P.rho,
+                                               (R)P.c, (R)P.h, (R)P.dt, nw, tz, tx);
+                    if (rc) return rc;
+                    HIPCHK(hipEventRecord(e1, ctx->stream));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    tmin = std::min(tmin, ms);
+                }
+                if (tmin < best) { best = tmin; pick = tz; pick_nw = nw; pick_tx = tx; }
             }
-            if (tmin < best) { best = tmin; pick = tz; pick_nw = nw; }
         }
     }
     (void)hipEventDestroy(e0);
@@ -368,6 +376,7 @@ int autotune_tile(fases_ctx *ctx) {
     HIPCHK(hipGetLastError());
     ctx->tz = pick;
     ctx->nw = pick_nw;
+    ctx->tx = pick_tx;
     return FASES_OK;
 }
 
@@ -432,7 +441,7 @@ int launch_stages(fases_ctx *ctx, long long n, const FieldSet<S> sets[4], double
                 hipLaunchKernelGGL((k_flux_simple<S, R, 2>), grd, blk, 0, ctx->stream, g, gio, ctx->d_upd, ctx->pml,
                                    rho, c, h, dt);
         } else {
-            rc = launch_tiled_nw<S, R>(ctx, s, sets[q.in], sets[0], sets[q.out], rho, c, h, dt, ctx->nw, ctx->tz);
+            rc = launch_tiled_nw<S, R>(ctx, s, sets[q.in], sets[0], sets[q.out], rho, c, h, dt, ctx->nw, ctx->tz, ctx->tx);
             if (rc) return rc;
         }
         if (kev) HIPCHK(hipEventRecord(e1, ctx->stream));

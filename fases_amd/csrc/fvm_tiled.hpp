@@ -1,12 +1,18 @@
 // Tiled flux-stage kernel for gfx950: WENO5-Z + Godunov + SSP-RK3 stage + cPML with
 // every face reconstructed once.
 //
-// Block = 64 lanes (x columns x0-1 .. x0+62) x NW waves (y rows j0 .. j0+NW-1),
-// marching over tz planes in z. Outputs are the inner 62 columns and all NW rows; the
-// outer lane of each side only supplies x-face states to its neighbour.
+// Block = 64 lanes x NW waves (y rows j0 .. j0+NW-1), marching over tz planes in z, in
+// one of two widths TX (picked per room by the autotuner; both give the same bits):
+//   TX = 64: lanes are x columns x0 .. x0+63, every lane an output; the fluxes through
+//            the tile's two x edges come from one wave that reconstructs the edge states
+//            (A of x0-1 and x0+63, B of x0 and x0+64, p and u of every row) for the block;
+//   TX = 62: lanes are x columns x0-1 .. x0+62, outputs the inner 62; the outer lanes
+//            only supply x-face states to their neighbours (no edge work, but 2 of 64
+//            lanes idle and more tiles on rooms whose width is a multiple of 64).
 //   z faces: register pipeline along the march (face k+1/2 computed once, carried);
-//   x faces: each lane reconstructs both states of its own cell, the right state of
-//            the next cell comes from the neighbouring lane (shuffle);
+//   x faces: each lane reconstructs both states of its own cell, the right state of the
+//            next cell comes from the neighbouring lane (shuffle), and so does the flux
+//            through the cell's left face;
 //   y faces: both states per cell written to LDS, read by the rows above and below; the
 //            one state the block needs of each halo row (left of j0-1, right of j0+NW, p
 //            and v) comes from four waves, 0, 1, NW-2, NW-1, i.e. one per SIMD.
@@ -80,6 +86,10 @@ __device__ inline R weno_left(const WenoG<R> &g, R d1, R d2, R d3, R d4, R c) {
     const R nl = fma(g.g0, a0, fma(g.g1, a1, g.g2 * a2));
     return fma(nl, fast_rcp(fma(R(0.3), g.g0, fma(R(0.6), g.g1, R(0.1) * g.g2))), c);
 }
+// (weno_right is not written as the operation-for-operation mirror of weno_left: that
+// order was measured to move the FP32 gradient of the 96x80x64 eps-level adjoint test
+// from 3e-4 to 1.1e-2 of max |g| against FP64, on the one wall whose gradient is most
+// sensitive to the forward's rounding.)
 template <typename R>
 __device__ inline R weno_right(const WenoG<R> &g, R d1, R d2, R d3, R d4, R c) {
     const R b0 = fma(R(-1.0 / 12.0), d3, R(1.0 / 30.0) * d4);
@@ -145,15 +155,21 @@ template <typename R> struct ZCarry {
 };
 
 namespace tile {
-constexpr int TXO = 62;         // output columns per block
-constexpr int LX = 68;          // LDS columns: x0-3 .. x0+64
+constexpr int XHW = 2;          // wave that reconstructs the x-edge states (TX = 64)
+// LDS columns of a tile of width TX: x0-3 .. x0+TX+2 (the own cell of lane l at column
+// l + halo_x: 3 for TX = 64, 2 for TX = 62)
+template <int TX> constexpr int lds_x() { return TX + 6; }
+template <int TX> constexpr int halo_x() { return (TX + 6 - 64) / 2; }
 }  // namespace tile
 
-template <typename R, int NW> struct TileLds {
-    R p[NW + 6][tile::LX];      // rows j0-3 .. j0+NW+2
-    R u[NW][tile::LX];
+template <typename R, int NW, int TX> struct TileLds {
+    R p[NW + 6][tile::lds_x<TX>()];      // rows j0-3 .. j0+NW+2
+    R u[NW][tile::lds_x<TX>()];
     R v[NW + 6][64];
     R ys[4][NW + 2][64];        // y states p^-, p^+, v^-, v^+ of rows j0-1 .. j0+NW
+    R xs[4][2][NW];             // x-edge states (p, u) of each row: A(x0-1), B(x0), A(x0+63), B(x0+64)
+    R xf[2][2][NW];             // x-edge fluxes (p, u) of each row through x0-1/2 and x0+63+1/2
+    R spare[64];                // target of the halo slots that carry no item
 };
 // Constants of the tiled kernel as unevaluated sums hi + lo (lo = 0 for FP64): in a
 // lossless resonant room a constant rounded once (dt/h, 1/(2 rho), 1/3) is a systematic
@@ -306,13 +322,15 @@ __device__ inline void pml_tile(const PmlSet &pml, const PmlLane<R> &PL, const P
     }
 }
 
-// One tile = 64 x NW threads; outputs 62 columns x NW rows x tz planes of tile
+// One tile = 64 x NW threads; outputs TX columns x NW rows x tz planes of tile
 // (bx, by, bz) of an Nx x Ny x Nz grid.
-template <typename S, typename R, int STAGE, int NW, bool PML_FUSED>
-__device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, const TileFields<S> &F,
+template <typename S, typename R, int STAGE, int NW, int TX, bool PML_FUSED>
+__device__ inline void flux_tile(TileLds<R, NW, TX> &L, int Nx, int Ny, int Nz, const TileFields<S> &F,
                                  const uint8_t *__restrict__ upd, const PmlSet *__restrict__ pmlp, const TileK<R> &K, R dt,
                                  const Scl<R> &sc, int tz, int bx, int by, int bz) {
     using namespace tile;
+    static_assert(TX == 64 || TX == 62, "tile width");
+    constexpr int LX = lds_x<TX>(), HX = halo_x<TX>();
     typedef Conv<S, R> CV;
     const S *__restrict__ ip = F.ip, *__restrict__ iu = F.iu, *__restrict__ iv = F.iv, *__restrict__ iw = F.iw;
     const S *__restrict__ bp = F.bp, *__restrict__ bu = F.bu, *__restrict__ bv = F.bv, *__restrict__ bw = F.bw;
@@ -320,15 +338,15 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     // blockDim.x is one wavefront, so the row index is uniform per wave: keep it scalar
     const int lx = threadIdx.x, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int tid = wy * 64 + lx;
-    const int x0 = 3 + TXO * bx, j0 = 3 + NW * by, k0 = 3 + tz * bz;
+    const int x0 = 3 + TX * bx, j0 = 3 + NW * by, k0 = 3 + tz * bz;
     const long long nxy = (long long)Nx * Ny;
     const int kend = min(k0 + tz, Nz - 3);
-    const int i = x0 - 1 + lx, j = j0 + wy;
+    const int i = x0 - 3 + HX + lx, j = j0 + wy;      // lane lx sits at LDS column lx + HX
     const int ic = min(i, Nx - 1), jc = min(j, Ny - 1);               // clamped for loads
     const unsigned col = (unsigned)(ic + Nx * jc);                   // offset within a plane
     const unsigned cb = col * (unsigned)sizeof(S);                   // ... in bytes
     const unsigned pbytes = (unsigned)nxy * (unsigned)sizeof(S);     // bytes of one plane
-    const bool out_lane = (lx >= 1 && lx <= TXO) && i < Nx - 3 && j < Ny - 3;
+    const bool out_lane = (TX == 64 || (lx >= 1 && lx <= 62)) && i < Nx - 3 && j < Ny - 3;
     // field value of this lane's column in the plane at element offset `pl`
     auto LD = [&](const S *f, long long pl, R scale) __device__ {
         return CV::ld(PlaneIO<S>::ld(plane_rsrc(f + pl, pbytes), cb), scale);
@@ -341,44 +359,59 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     };
 
     // halo items of this thread: items tid, tid + 64 NW, ... of
-    //   p: rows 0-2 and NW+3..NW+5 (all LX columns), columns 0, 1, LX-2, LX-1 of the own rows;
-    //   u: columns 0, 1, LX-2, LX-1 of the own rows;  v: rows 0-2 and NW+3..NW+5
-    constexpr int nP1 = 6 * LX, nP2 = 4 * NW, nU = 4 * NW, nV = 6 * 64;
-    constexpr int NITEM = nP1 + nP2 + nU + nV;
-    constexpr int NH = (NITEM + 64 * NW - 1) / (64 * NW);
-    bool h_on[NH];
-    const S *hsrc[NH];
-    R hsc[NH];
+    //   p: rows 0-2 and NW+3..NW+5 (all LX columns), the HX columns of each side of the own
+    //   rows;  u: the same side columns of the own rows;  v: rows 0-2 and NW+3..NW+5.
+    // Each field's items start on a multiple of 64, so the field of a slot is uniform over
+    // a wave: its loads are buffer loads on the plane descriptor (scalar base, per-lane
+    // offset). A slot without an item loads at the plane's end (the hardware returns 0)
+    // and stores into the spare row.
+    constexpr int nP1 = 6 * LX, nP2 = 2 * HX * NW, nU = 2 * HX * NW, nV = 6 * 64;
+    constexpr int kPEnd = (nP1 + nP2 + 63) / 64 * 64, kUEnd = kPEnd + (nU + 63) / 64 * 64, kVEnd = kUEnd + nV;
+    constexpr int NH = (kVEnd + 64 * NW - 1) / (64 * NW);
+    int h_kind[NH];             // 0 p, 1 u, 2 v, 3 none (wave-uniform)
+    const S *hsrc[NH];          // its field (wave-uniform, scalar registers)
+    R hsc[NH];                  // and scale
     R *hdst[NH];
-    unsigned h_off[NH];
+    unsigned h_off[NH];         // byte offset within a plane
 #pragma unroll
     for (int q = 0; q < NH; ++q) {
-        const int sideX[4] = {0, 1, LX - 2, LX - 1};
+        const int sideX[6] = {0, 1, HX == 3 ? 2 : LX - 2, HX == 3 ? LX - 3 : LX - 1, LX - 2, LX - 1};
         const int sideY[6] = {0, 1, 2, NW + 3, NW + 4, NW + 5};
-        int t = tid + q * 64 * NW;
-        int h_kind = -1, h_r = 0, h_c = 0;
-        if (t < nP1) {
-            h_kind = 0; h_r = sideY[t / LX]; h_c = t % LX;
-        } else if ((t -= nP1) < nP2) {
-            h_kind = 0; h_c = sideX[t / NW]; h_r = 3 + t % NW;
-        } else if ((t -= nP2) < nU) {
-            h_kind = 1; h_c = sideX[t / NW]; h_r = t % NW;
-        } else if ((t -= nU) < nV) {
-            h_kind = 2; h_r = sideY[t / 64]; h_c = t % 64;
+        const int t = tid + q * 64 * NW;
+        const int tw = __builtin_amdgcn_readfirstlane(t);
+        const int kind = tw < kPEnd ? 0 : (tw < kUEnd ? 1 : (tw < kVEnd ? 2 : 3));
+        int h_r = -1, h_c = 0;
+        if (kind == 0) {
+            if (t < nP1) {
+                h_r = sideY[t / LX]; h_c = t % LX;
+            } else if (t - nP1 < nP2) {
+                h_c = sideX[(t - nP1) / NW]; h_r = 3 + (t - nP1) % NW;
+            }
+        } else if (kind == 1) {
+            if (t - kPEnd < nU) { h_c = sideX[(t - kPEnd) / NW]; h_r = (t - kPEnd) % NW; }
+        } else if (kind == 2) {
+            h_r = sideY[(t - kUEnd) / 64]; h_c = (t - kUEnd) % 64;
         }
-        h_on[q] = h_kind >= 0;
-        h_off[q] = 0;
-        if (h_kind >= 0) {
-            int gx = (h_kind == 2) ? (x0 - 1 + h_c) : (x0 - 3 + h_c);
-            int gy = (h_kind == 1) ? (j0 + h_r) : (j0 - 3 + h_r);
+        h_kind[q] = kind;
+        {
+            const unsigned long long a = (unsigned long long)(kind == 0 ? ip : (kind == 1 ? iu : iv));
+            const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+            const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+            hsrc[q] = (const S *)(((unsigned long long)hi << 32) | lo);
+            hsc[q] = kind == 0 ? sc.s[0] : (kind == 1 ? sc.s[1] : sc.s[2]);
+        }
+        h_off[q] = (unsigned)nxy * (unsigned)sizeof(S);             // out of range: reads 0
+        hdst[q] = &L.spare[lx];
+        if (h_r >= 0) {
+            int gx = (kind == 2) ? (x0 - 3 + HX + h_c) : (x0 - 3 + h_c);
+            int gy = (kind == 1) ? (j0 + h_r) : (j0 - 3 + h_r);
             gx = max(0, min(gx, Nx - 1));
             gy = max(0, min(gy, Ny - 1));
-            h_off[q] = (unsigned)(gx + Nx * gy);
+            h_off[q] = (unsigned)(gx + Nx * gy) * (unsigned)sizeof(S);
+            hdst[q] = (kind == 0) ? &L.p[h_r][h_c] : (kind == 1 ? &L.u[h_r][h_c] : &L.v[h_r][h_c]);
         }
-        hsrc[q] = (h_kind == 0) ? ip : (h_kind == 1 ? iu : iv);
-        hsc[q] = (h_kind == 0) ? sc.s[0] : (h_kind == 1 ? sc.s[1] : sc.s[2]);
-        hdst[q] = (h_kind == 0) ? &L.p[h_r][h_c] : (h_kind == 1 ? &L.u[h_r][h_c] : &L.v[h_r][h_c]);
     }
+    auto LDH = [&](int q, long long pl) __device__ { return PlaneIO<S>::ld(plane_rsrc(hsrc[q] + pl, pbytes), h_off[q]); };
 
     // z pipeline: carried windows of p and w (centres k..k+2, differences, smoothness
     // terms), the newest plane k+3 (pn, wn), the A-state of cell k and the flux through k-1/2
@@ -416,7 +449,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         nflag = plane_ld_u8(upd + pl, (unsigned)nxy, col);
 #pragma unroll
         for (int q = 0; q < NH; ++q)                       // per-lane field: no uniform base
-            nh_s[q] = h_on[q] ? (hsrc[q] + pl)[h_off[q]] : S();
+            if (h_kind[q] != 3) nh_s[q] = LDH(q, pl);
         const long long p3 = nxy * (long long)(k0 + 3);
         pend_p = LDR(ip, p3);
         pend_w = LDR(iw, p3);
@@ -435,7 +468,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
     if constexpr (PML_FUSED) {
         const PmlSet &pm = *pmlp;
         if (pm.on) {
-            const bool pml_xy = x0 < pm.clear0[0] || x0 + TXO > pm.clear1[0] || j0 < pm.clear0[1] ||
+            const bool pml_xy = x0 < pm.clear0[0] || x0 + TX > pm.clear1[0] || j0 < pm.clear0[1] ||
                                 j0 + NW > pm.clear1[1];
             pz_lo = pml_xy ? INT_MAX : pm.clear0[2];    // every plane / the z bands
             pz_hi = pml_xy ? INT_MIN : pm.clear1[2];
@@ -469,7 +502,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         const PmlPsi<R> psi_c = psi_n;
         R hv[NH];
 #pragma unroll
-        for (int q = 0; q < NH; ++q) hv[q] = h_on[q] ? CV::ld(nh_s[q], hsc[q]) : R(0);
+        for (int q = 0; q < NH; ++q) hv[q] = h_kind[q] != 3 ? CV::ld(nh_s[q], hsc[q]) : R(0);
         // ---- the next plane's loads (p, w of plane k+4 for the z window)
         auto issue_next = [&]() __device__ __attribute__((always_inline)) {
             if (k + 1 >= kend) return;
@@ -479,7 +512,7 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
             nflag = plane_ld_u8(upd + pn, (unsigned)nxy, col);
 #pragma unroll
             for (int q = 0; q < NH; ++q)
-                if (h_on[q]) nh_s[q] = (hsrc[q] + pn)[h_off[q]];
+                if (h_kind[q] != 3) nh_s[q] = LDH(q, pn);
             const long long p4 = nxy * (long long)(k + 4);
             pend_p = LDR(ip, p4);
             pend_w = LDR(iw, p4);
@@ -487,25 +520,30 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
         };
         if constexpr (kRaw) issue_next();
         // ---- fill the plane tiles
-        L.p[wy + 3][lx + 2] = pc;
-        L.u[wy][lx + 2] = uc;
+        L.p[wy + 3][lx + HX] = pc;
+        L.u[wy][lx + HX] = uc;
         L.v[wy + 3][lx] = vc;
 #pragma unroll
         for (int q = 0; q < NH; ++q)
-            if (h_on[q]) *hdst[q] = hv[q];
+            if (h_kind[q] != 3) *hdst[q] = hv[q];
         __syncthreads();
         if constexpr (!kRaw) issue_next();
-        S b0 = S(), b1 = S(), b2 = S(), b3 = S();        // RK base set of this plane
-        if (STAGE != 0) {
-            b0 = LDR(bp, pl);
-            b1 = LDR(bu, pl);
-            b2 = LDR(bv, pl);
-            b3 = LDR(bw, pl);
-        }
+        // RK base set of this plane: loaded here (TX = 62) or after the second barrier (TX =
+        // 64, whose edge work would otherwise keep the four values live across more code)
+        S b0 = S(), b1 = S(), b2 = S(), b3 = S();
+        auto load_base = [&]() __device__ __attribute__((always_inline)) {
+            if (STAGE != 0) {
+                b0 = LDR(bp, pl);
+                b1 = LDR(bu, pl);
+                b2 = LDR(bv, pl);
+                b3 = LDR(bw, pl);
+            }
+        };
+        if constexpr (TX == 62) load_base();
         // ---- y states of the own row, and of the halo rows on four waves
         {
             R A, B, Av, Bv;
-            weno_pair(L.p[wy + 1][lx + 2], L.p[wy + 2][lx + 2], pc, L.p[wy + 4][lx + 2], L.p[wy + 5][lx + 2], A, B);
+            weno_pair(L.p[wy + 1][lx + HX], L.p[wy + 2][lx + HX], pc, L.p[wy + 4][lx + HX], L.p[wy + 5][lx + HX], A, B);
             weno_pair(L.v[wy + 1][lx], L.v[wy + 2][lx], vc, L.v[wy + 4][lx], L.v[wy + 5][lx], Av, Bv);
             L.ys[0][wy + 1][lx] = A;
             L.ys[1][wy + 1][lx] = B;
@@ -514,32 +552,63 @@ __device__ inline void flux_tile(TileLds<R, NW> &L, int Nx, int Ny, int Nz, cons
             // halo rows: the block needs the left state of row j0-1 (face j0-1/2) and the
             // right state of row j0+NW (face j0+NW-1/2), one reconstruction per SIMD
             if (wy == 0) {                           // p of row j0-1 (ys row 0, L.p rows 0..4)
-                L.ys[0][0][lx] = weno_left_of(L.p[0][lx + 2], L.p[1][lx + 2], L.p[2][lx + 2], L.p[3][lx + 2],
-                                              L.p[4][lx + 2]);
+                L.ys[0][0][lx] = weno_left_of(L.p[0][lx + HX], L.p[1][lx + HX], L.p[2][lx + HX], L.p[3][lx + HX],
+                                              L.p[4][lx + HX]);
             } else if (wy == 1) {                    // v of row j0-1
                 L.ys[2][0][lx] = weno_left_of(L.v[0][lx], L.v[1][lx], L.v[2][lx], L.v[3][lx], L.v[4][lx]);
             } else if (wy == NW - 2) {               // p of row j0+NW (ys row NW+1, L.p rows NW+1..NW+5)
                 constexpr int r = NW + 1;
-                L.ys[1][r][lx] = weno_right_of(L.p[r][lx + 2], L.p[r + 1][lx + 2], L.p[r + 2][lx + 2],
-                                               L.p[r + 3][lx + 2], L.p[r + 4][lx + 2]);
+                L.ys[1][r][lx] = weno_right_of(L.p[r][lx + HX], L.p[r + 1][lx + HX], L.p[r + 2][lx + HX],
+                                               L.p[r + 3][lx + HX], L.p[r + 4][lx + HX]);
             } else if (wy == NW - 1) {               // v of row j0+NW
                 constexpr int r = NW + 1;
                 L.ys[3][r][lx] = weno_right_of(L.v[r][lx], L.v[r + 1][lx], L.v[r + 2][lx], L.v[r + 3][lx],
                                                L.v[r + 4][lx]);
+            } else if (TX == 64 && wy == XHW) {
+                // x-edge faces of every row, x0-1/2 and x0+63+1/2: the four states
+                // A(x0-1), B(x0), A(x0+63), B(x0+64) of p and u (weno_pair of the cell's
+                // window, the same bits as the rows' own lanes), then the two fluxes, for
+                // the rows' lanes 0 and 63
+#pragma unroll
+                for (int t = lx; t < 8 * NW; t += 64) {
+                    const int r = t % NW, var = (t / NW) & 1, st = t / (2 * NW);
+                    // window columns 0..4, 1..5, 64..68, 65..69; st odd: the right state
+                    const R *w = (var ? &L.u[r][0] : &L.p[r + 3][0]) + ((st & 2) ? LX - 6 : 0) + (st & 1);
+                    R A, B;
+                    weno_pair(w[0], w[1], w[2], w[3], w[4], A, B);
+                    L.xs[st][var][r] = (st & 1) ? B : A;
+                }
+                __builtin_amdgcn_wave_barrier();    // LDS operations of one wave complete in order
+                if (lx < 2 * NW) {
+                    const int r = lx % NW, f = lx / NW;
+                    R Fp, Fu;
+                    godunov_f(L.xs[2 * f][0][r], L.xs[2 * f + 1][0][r], L.xs[2 * f][1][r], L.xs[2 * f + 1][1][r], K, Fp, Fu);
+                    L.xf[f][0][r] = Fp;
+                    L.xf[f][1][r] = Fu;
+                }
             }
         }
-        // ---- x faces (shuffles are wave-wide: every lane takes part)
+        // ---- x faces (shuffles are wave-wide: every lane takes part); with TX = 64, lane
+        // 63's face i+1/2 and lane 0's face i-1/2 are the edge wave's fluxes
         R FxP_p, FxP_u, FxM_p, FxM_u;
         {
             R A, B, Au, Bu;
-            weno_pair(L.p[wy + 3][lx], L.p[wy + 3][lx + 1], pc, L.p[wy + 3][lx + 3], L.p[wy + 3][lx + 4], A, B);
-            weno_pair(L.u[wy][lx], L.u[wy][lx + 1], uc, L.u[wy][lx + 3], L.u[wy][lx + 4], Au, Bu);
+            weno_pair(L.p[wy + 3][lx + HX - 2], L.p[wy + 3][lx + HX - 1], pc, L.p[wy + 3][lx + HX + 1],
+                      L.p[wy + 3][lx + HX + 2], A, B);
+            weno_pair(L.u[wy][lx + HX - 2], L.u[wy][lx + HX - 1], uc, L.u[wy][lx + HX + 1], L.u[wy][lx + HX + 2], Au, Bu);
             const R Bn = shfl_next(B), Bun = shfl_next(Bu);
             godunov_f(A, Bn, Au, Bun, K, FxP_p, FxP_u);          // face i+1/2
             FxM_p = shfl_prev(FxP_p);
             FxM_u = shfl_prev(FxP_u);
         }
         __syncthreads();
+        if constexpr (TX == 64) load_base();
+        if constexpr (TX == 64) {
+            FxM_p = lx == 0 ? L.xf[0][0][wy] : FxM_p;
+            FxM_u = lx == 0 ? L.xf[0][1][wy] : FxM_u;
+            FxP_p = lx == 63 ? L.xf[1][0][wy] : FxP_p;
+            FxP_u = lx == 63 ? L.xf[1][1][wy] : FxP_u;
+        }
 
         // ---- y faces j+1/2 and j-1/2
         R FyP_p, FyP_v, FyM_p, FyM_v;
@@ -629,16 +698,16 @@ __device__ inline unsigned xcd_tile(unsigned w, unsigned n) {
 // with two blocks per CU at 84-96 VGPRs. Stages 1 and 2 fit because their RK base set is
 // loaded in its own plane, after the first barrier, rather than a plane ahead (ahead, the
 // four values stay live across the whole plane: 19-27 spills at 80 VGPRs, 905 us).
-template <typename S, typename R, int STAGE, int NW>
+template <typename S, typename R, int STAGE, int NW, int TX>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 6 : 1, 8)))
 k_flux_tiled(int Nx, int Ny, int Nz, TileFields<S> F, const uint8_t *__restrict__ upd, const PmlSet *__restrict__ pml,
              TileK<R> K, R dt,
              Scl<R> sc, int tz, int nbx, int nby) {
-    __shared__ TileLds<R, NW> L;
+    __shared__ TileLds<R, NW, TX> L;
     const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
     const int bx = (int)(t % (unsigned)nbx), by = (int)((t / (unsigned)nbx) % (unsigned)nby);
     const int bz = (int)(t / ((unsigned)nbx * (unsigned)nby));
-    flux_tile<S, R, STAGE, NW, false>(L, Nx, Ny, Nz, F, upd, pml, K, dt, sc, tz, bx, by, bz);
+    flux_tile<S, R, STAGE, NW, TX, false>(L, Nx, Ny, Nz, F, upd, pml, K, dt, sc, tz, bx, by, bz);
 }
 
 // cPML of one stage on the packed list of band cells, after k_flux_tiled (which leaves

@@ -65,6 +65,7 @@ int fases_params_default(fases_params *p) {
     p->tile_z = 0;
     p->adjoint_variant = 0;
     p->tile_rows = 0;
+    p->tile_cols = 0;
     p->source_patch = 0;
     return FASES_OK;
 }

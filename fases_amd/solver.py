@@ -268,6 +268,7 @@ class SolverParams:
     adjoint_variant: int = 0            # flux adjoint: 0 tiled, 1 per cell, 2 tiled in XCD-aware order
     tile_rows: int = 0                  # tiled kernel waves per block (8 or 16); 0 = autotune
     source_patch: int = 0               # FP64 radius around sources under FP32/FP16 storage; 0 = 12
+    tile_cols: int = 0                  # tiled kernel columns per block (64 or 62); 0 = autotune
 
     def to_c(self) -> _lib.fases_params:
         p = _lib.fases_params()
@@ -288,6 +289,7 @@ class SolverParams:
         p.adjoint_variant = int(self.adjoint_variant)
         p.tile_rows = int(self.tile_rows)
         p.source_patch = int(self.source_patch)
+        p.tile_cols = int(self.tile_cols)
         return p
 
 

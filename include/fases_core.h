@@ -83,6 +83,8 @@ typedef struct fases_params {
                                each point source (forward and adjoint); 0 = 12. Drive
                                gradients of large rooms need ~32 (FP32 drive gradient of
                                the 192x160x128 room: 14 % off FP64 at 12, 8e-4 at 32)     */
+    int    tile_cols;       /* tiled kernel: x columns per block, 64 or 62 (same results);
+                               0 = autotune together with tile_z and tile_rows              */
 } fases_params;
 
 /* Fills *p with the reference defaults (FP64, device 0, PML depth 20, ...). */
