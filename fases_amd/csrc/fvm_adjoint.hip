@@ -14,7 +14,12 @@
 // them); the reverse sweep adds the PML part of the flux adjoint and carries the adjoint
 // of the auxiliary states from step to step (k_pml_flux_adj, k_pml_psi_adj).
 //
-// Scope: FP32 or FP64 field storage.
+// FP16 storage: the forward rescales the fields at the steps the plain run does and keeps
+// each step's scale exponents; the segment recompute applies the same exact power-of-two
+// rescale to the step's start set, and every adjoint kernel of a step reads that step's
+// stored stage inputs with that step's scales.
+//
+// Scope: FP16, FP32 or FP64 field storage.
 
 #include "fvm_adjoint.hpp"
 #include "fvm_ctx.hpp"
@@ -50,6 +55,8 @@ struct AdjBufs {
     std::vector<void *> ck_f[4];
     std::vector<double *> ck_sh;
     std::vector<AdeState> ck_x;
+    // FP16 storage: the field scale exponents in effect for each step's stages [nsteps][4]
+    std::vector<int> fexp;
     // segment slots: V0[0..every], V1[0..every-1], V2[0..every-1] (+ shadows), ADE x0 per step
     std::vector<void *> v0[4], v1[4], v2[4];
     std::vector<double *> s0, s1, s2;
@@ -403,7 +410,13 @@ int forward_impl(fases_ctx *ctx, long long nsteps, int every, double *ir) {
     if ((rc = prepare_run(ctx, nsteps))) return rc;
     if (uses_tiled(ctx) && ctx->tz == 0 && (rc = autotune_tile<S, R>(ctx))) return rc;
     const size_t N = (size_t)ctx->N, np = (size_t)std::max(ctx->n_patch, 0);
+    constexpr bool kHalf = std::is_same<S, __half>::value;
+    A->fexp.assign(kHalf ? (size_t)nsteps * 4 : 0, 0);
     for (long long q = 0; q < nsteps; ++q) {
+        if (kHalf) {                    // the plain run's rescale schedule (run_impl)
+            if (rescale_due(ctx->step, ctx->prm.fp16_rescale_every) && (rc = rescale_fields<S>(ctx))) return rc;
+            std::copy(ctx->fexp, ctx->fexp + 4, A->fexp.begin() + 4 * q);
+        }
         if (q % every == 0) {
             const int c = (int)(q / every);
             for (int f = 0; f < 4; ++f)
@@ -441,8 +454,18 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
     const long long nsteps = A.nsteps;
     const size_t N = (size_t)ctx->N, np = (size_t)std::max(ctx->n_patch, 0);
     const Grid g{ctx->Nx, ctx->Ny, ctx->Nz, (long long)ctx->Nx * ctx->Ny};
-    const Scl<R> sc = field_scales<R>(ctx);
-    const Scl<double> scd = field_scales<double>(ctx);
+    Scl<R> sc = field_scales<R>(ctx);
+    Scl<double> scd = field_scales<double>(ctx);
+    // FP16 storage: step n's scale exponents (the forward's history) into the context
+    constexpr bool kHalf = std::is_same<S, __half>::value;
+    int fexp_end[4];
+    std::copy(ctx->fexp, ctx->fexp + 4, fexp_end);
+    auto step_scales = [&](long long n) {
+        if (!kHalf) return;
+        std::copy(A.fexp.begin() + 4 * (n - A.step0), A.fexp.begin() + 4 * (n - A.step0) + 4, ctx->fexp);
+        sc = field_scales<R>(ctx);
+        scd = field_scales<double>(ctx);
+    };
     const Ade ade{ctx->K, ctx->L, ctx->d_Rinf, ctx->d_A, ctx->d_zeta, ctx->d_B, ctx->d_C, ctx->d_alpha, ctx->d_beta,
                   ctx->d_Keff, ctx->d_Leff};
     const Bnd<double> bd{ctx->nB, ctx->d_bcell, ctx->d_bbase, ctx->d_bmat, ctx->d_bgeo};
@@ -511,6 +534,22 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
             if ((rc = copy_ade(ctx, A.xs[(size_t)q], ctx->phi[1], ctx->psi1[1], ctx->psi2[1]))) return rc;
             const FieldSet<S> sets[4] = {fs_of<S>(A.v0, (size_t)q), fs_of<S>(A.v1, (size_t)q),
                                          fs_of<S>(A.v2, (size_t)q), fs_of<S>(A.v0, (size_t)q + 1)};
+            if (kHalf && q > 0) {
+                // the forward's rescale at the start of this step: the same exact power-of-two
+                // factors on the step's start set (the checkpoint of a segment's first step
+                // was taken after it)
+                const int *pe = &A.fexp[4 * (size_t)(n0 + q - 1 - A.step0)], *ce = &A.fexp[4 * (size_t)(n0 + q - A.step0)];
+                if (!std::equal(pe, pe + 4, ce)) {
+                    Scl<float> fac;
+                    for (int f = 0; f < 4; ++f) {
+                        fac.s[f] = (float)std::ldexp(1.0, pe[f] - ce[f]);
+                        fac.is[f] = 1.0f / fac.s[f];
+                    }
+                    const int nb = (int)std::min<long long>(4096, (ctx->N + 255) / 256);
+                    hipLaunchKernelGGL((k_rescale<S>), dim3(nb), dim3(256), 0, ctx->stream, sets[0], ctx->N, fac);
+                }
+            }
+            step_scales(n0 + q);
             double *const sh[4] = {A.s0[(size_t)q], np ? A.s1[(size_t)q] : nullptr, np ? A.s2[(size_t)q] : nullptr,
                                    A.s0[(size_t)q + 1]};
             if ((rc = launch_stages<S, R>(ctx, n0 + q, sets, sh, nullptr))) return rc;
@@ -521,6 +560,7 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
         // adjoint steps of the segment, last first
         for (long long q = len - 1; q >= 0; --q) {
             const long long n = n0 + q;
+            step_scales(n);
             const FieldSet<S> V0 = fs_of<S>(A.v0, (size_t)q), V1 = fs_of<S>(A.v1, (size_t)q),
                               V2 = fs_of<S>(A.v2, (size_t)q);
             const AdeState &X0 = A.xs[(size_t)q];
@@ -636,6 +676,7 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipGetLastError());
+    std::copy(fexp_end, fexp_end + 4, ctx->fexp);
     // material gradients: sum the per-cell contributions by material
     if (grad_mat) {
         std::vector<double> Gh((size_t)ctx->nB * A.P);
@@ -663,10 +704,9 @@ int fases_adjoint_forward(fases_ctx *ctx, int64_t nsteps, int checkpoint_every, 
         return set_error(ctx, FASES_E_STATE, "fases_adjoint_forward: geometry and materials must be set first");
     if (nsteps < 1 || checkpoint_every < 1)
         return set_error(ctx, FASES_E_ARG, "fases_adjoint_forward: nsteps and checkpoint_every must be >= 1");
-    if (ctx->prm.precision == FASES_FP16)
-        return set_error(ctx, FASES_E_ARG, "fases_adjoint_forward: FP16 storage has no adjoint; use FP32 or FP64");
     HIPCHK(hipSetDevice(ctx->prm.device));
     if (ctx->prm.precision == FASES_FP64) return forward_impl<double, double>(ctx, nsteps, checkpoint_every, ir);
+    if (ctx->prm.precision == FASES_FP16) return forward_impl<__half, float>(ctx, nsteps, checkpoint_every, ir);
     return forward_impl<float, float>(ctx, nsteps, checkpoint_every, ir);
 }
 
@@ -676,6 +716,7 @@ int fases_adjoint(fases_ctx *ctx, const double *dJ_dir, double *grad_mat, double
     if (ctx->n_rec > 0 && !dJ_dir) return set_error(ctx, FASES_E_ARG, "fases_adjoint: dJ_dir is NULL");
     HIPCHK(hipSetDevice(ctx->prm.device));
     if (ctx->prm.precision == FASES_FP64) return backward_impl<double, double>(ctx, dJ_dir, grad_mat, grad_sval, grad_us);
+    if (ctx->prm.precision == FASES_FP16) return backward_impl<__half, float>(ctx, dJ_dir, grad_mat, grad_sval, grad_us);
     return backward_impl<float, float>(ctx, dJ_dir, grad_mat, grad_sval, grad_us);
 }
 

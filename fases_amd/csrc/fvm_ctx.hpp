@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -378,6 +379,56 @@ int autotune_tile(fases_ctx *ctx) {
     ctx->nw = pick_nw;
     ctx->tx = pick_tx;
     return FASES_OK;
+}
+
+// FP16 scale rule: the largest |value| of a field is stored near 2^kHalfTop, leaving a
+// 2^(15 - kHalfTop) headroom to the FP16 maximum (65504) for growth until the next rescale
+// and 2^(kHalfTop + 14) below it before values turn subnormal.
+constexpr int kHalfTop = 6;
+
+int exponent_for(double amax, int keep) {
+    if (!(amax > 0.0)) return keep;
+    int e;
+    (void)std::frexp(amax, &e);     // amax in [2^(e-1), 2^e)
+    return e - kHalfTop;
+}
+
+template <typename S>
+int rescale_fields(fases_ctx *ctx) {
+    if (!ctx->d_amax) {
+        int rc = dalloc(ctx, &ctx->d_amax, 5);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL((k_fill<unsigned>), dim3(1), dim3(64), 0, ctx->stream, ctx->d_amax, 5, 0u);
+    const int nb = (int)std::min<long long>(4096, (ctx->N + 255) / 256);
+    hipLaunchKernelGGL((k_absmax<S>), dim3(nb), dim3(256), 0, ctx->stream, fset<S>(ctx, 0), ctx->N, ctx->d_amax);
+    unsigned hm[5];
+    HIPCHK(hipMemcpyAsync(hm, ctx->d_amax, sizeof hm, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (hm[4]) return set_error(ctx, FASES_E_NONFINITE, "FP16 field storage overflowed (non-finite value)");
+    Scl<float> fac;
+    bool change = false;
+    for (int f = 0; f < 4; ++f) {
+        float m;
+        std::memcpy(&m, &hm[f], sizeof m);
+        const int e = exponent_for(std::ldexp((double)m, ctx->fexp[f]), ctx->fexp[f]);
+        fac.s[f] = (float)std::ldexp(1.0, ctx->fexp[f] - e);
+        fac.is[f] = 1.0f / fac.s[f];
+        change |= (e != ctx->fexp[f]);
+        ctx->fexp[f] = e;
+    }
+    if (change)
+        for (int q = 0; q < 3; ++q)
+            hipLaunchKernelGGL((k_rescale<S>), dim3(nb), dim3(256), 0, ctx->stream, fset<S>(ctx, q), ctx->N, fac);
+    return FASES_OK;
+}
+
+// Steps (global index) at whose start an FP16 run rescales: 1, 2, 4, ... below the
+// period (fields grow fastest from rest), then every period.
+bool rescale_due(long long n, int period) {
+    if (period <= 0 || n == 0) return false;
+    if (n < period) return (n & (n - 1)) == 0;
+    return n % period == 0;
 }
 
 // Stage-collapsed ADE parameters and buffers of a context (reduced-precision storage):

@@ -264,9 +264,11 @@ int fases_run(fases_ctx *ctx, int64_t nsteps, double *ir, double *vel,
 
 /* Forward run of nsteps steps from the current state that also stores a checkpoint
  * (fields, ADE states, cPML auxiliary states, FP64 patch shadows) every
- * checkpoint_every steps for fases_adjoint. ir as in fases_run. FP32 or FP64 storage
- * (FP16 is rejected). Checkpoint and segment buffers stay allocated in the
- * context and are reused by the next call of the same shape.
+ * checkpoint_every steps for fases_adjoint. ir as in fases_run. Any storage precision;
+ * with FP16 the fields are rescaled at the steps fases_run rescales them and every
+ * step's scale exponents are kept for the reverse sweep. Checkpoint and segment
+ * buffers stay allocated in the context and are reused by the next call of the same
+ * shape.
  * No counterpart in the reference (gradients there come from fitting loops outside the
  * solver, fminconFIT.py / vectFIT.py driving repeated runs). */
 int fases_adjoint_forward(fases_ctx *ctx, int64_t nsteps, int checkpoint_every, double *ir);
