@@ -554,6 +554,8 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
         if (patch.empty()) {                       // no source any more: drop the old patch
             if ((rc = dfree(ctx, ctx->d_patch))) return rc;
             ctx->d_patch = nullptr;
+            if ((rc = dfree(ctx, ctx->d_patch_fx))) return rc;
+            ctx->d_patch_fx = nullptr;
             for (auto &sh : ctx->shadow) {
                 if ((rc = dfree(ctx, sh))) return rc;
                 sh = nullptr;
@@ -566,6 +568,12 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
                                   ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));   // pidx ends with this block
             if ((rc = upload_replace(ctx, &ctx->d_patch, patch.data(), patch.size()))) return rc;
+            if (patch.size() < (size_t)kPatchSplitBelow) {       // face fluxes of the two-launch update
+                if ((rc = dalloc_replace(ctx, &ctx->d_patch_fx, (size_t)12 * patch.size()))) return rc;
+            } else {
+                if ((rc = dfree(ctx, ctx->d_patch_fx))) return rc;
+                ctx->d_patch_fx = nullptr;
+            }
             for (int q = 0; q < 3; ++q) {
                 if ((rc = dalloc_replace(ctx, &ctx->shadow[q], (size_t)4 * patch.size()))) return rc;
                 if (ctx->esz == 4)

@@ -69,34 +69,35 @@ template <typename S> struct PatchIO {
     }
 };
 
-template <typename IO, typename R, int STAGE>
-__device__ inline void flux_cell(const Grid &g, int i, int j, int k, const IO &io, const PmlSet &pml, R rho, R c,
-                                 R h, R dt) {
+// Fluxes through the faces C+1/2 (P) and C-1/2 (M) of cell (i, j, k) along axis a (stride
+// st): pressure (Fp_p, Fp_m) and the axis velocity fu (Fu_p, Fu_m).
+template <typename IO, typename R>
+__device__ inline void flux_axis_cell(const Grid &g, int i, int j, int k, const IO &io, int fu, long long st, R rho,
+                                      R c, R &Fp_p, R &Fp_m, R &Fu_p, R &Fu_m) {
 #pragma clang fp contract(off)
     const long long C = (long long)i + (long long)g.Nx * j + g.nxy * k;
-
-    // fluxes through the faces C+1/2 (P) and C-1/2 (M) along one axis
-    auto flux_axis = [&](int fu, long long st, R &Fp_p, R &Fp_m, R &Fu_p, R &Fu_m) {
-        R pw[7], uw[7];
+    R pw[7], uw[7];
 #pragma unroll
-        for (int q = 0; q < 7; ++q) {
-            pw[q] = io.ld_in(0, C + (q - 3) * st);
-            uw[q] = io.ld_in(fu, C + (q - 3) * st);
-        }
-        R pl, pr, ul, ur;
-        weno_face(pw[1], pw[2], pw[3], pw[4], pw[5], pw[6], pl, pr);   // face +1/2
-        weno_face(uw[1], uw[2], uw[3], uw[4], uw[5], uw[6], ul, ur);
-        godunov(pl, pr, ul, ur, rho, c, Fp_p, Fu_p);
-        weno_face(pw[0], pw[1], pw[2], pw[3], pw[4], pw[5], pl, pr);   // face -1/2
-        weno_face(uw[0], uw[1], uw[2], uw[3], uw[4], uw[5], ul, ur);
-        godunov(pl, pr, ul, ur, rho, c, Fp_m, Fu_m);
-    };
+    for (int q = 0; q < 7; ++q) {
+        pw[q] = io.ld_in(0, C + (q - 3) * st);
+        uw[q] = io.ld_in(fu, C + (q - 3) * st);
+    }
+    R pl, pr, ul, ur;
+    weno_face(pw[1], pw[2], pw[3], pw[4], pw[5], pw[6], pl, pr);   // face +1/2
+    weno_face(uw[1], uw[2], uw[3], uw[4], uw[5], uw[6], ul, ur);
+    godunov(pl, pr, ul, ur, rho, c, Fp_p, Fu_p);
+    weno_face(pw[0], pw[1], pw[2], pw[3], pw[4], pw[5], pl, pr);   // face -1/2
+    weno_face(uw[0], uw[1], uw[2], uw[3], uw[4], uw[5], ul, ur);
+    godunov(pl, pr, ul, ur, rho, c, Fp_m, Fu_m);
+}
 
-    R FpP, FpM, FuP, FuM, GpP, GpM, GvP, GvM, HpP, HpM, HwP, HwM;
-    flux_axis(1, 1, FpP, FpM, FuP, FuM);
-    flux_axis(2, g.Nx, GpP, GpM, GvP, GvM);
-    flux_axis(3, g.nxy, HpP, HpM, HwP, HwM);
-
+// Stage update of cell (i, j, k) from its twelve face fluxes (x: F, y: G, z: H).
+template <typename IO, typename R, int STAGE>
+__device__ inline void flux_cell_update(const Grid &g, int i, int j, int k, const IO &io, const PmlSet &pml, R h,
+                                        R dt, R FpP, R FpM, R FuP, R FuM, R GpP, R GpM, R GvP, R GvM, R HpP, R HpM,
+                                        R HwP, R HwM) {
+#pragma clang fp contract(off)
+    const long long C = (long long)i + (long long)g.Nx * j + g.nxy * k;
     const R p_in = io.ld_in(0, C), u_in = io.ld_in(1, C), v_in = io.ld_in(2, C), w_in = io.ld_in(3, C);
     const R sp = -FpP + FpM - GpP + GpM - HpP + HpM;
     R uo, vo, wo, po, wgt;
@@ -131,6 +132,18 @@ __device__ inline void flux_cell(const Grid &g, int i, int j, int k, const IO &i
     io.st(0, C, po);
 }
 
+
+template <typename IO, typename R, int STAGE>
+__device__ inline void flux_cell(const Grid &g, int i, int j, int k, const IO &io, const PmlSet &pml, R rho, R c,
+                                 R h, R dt) {
+    R FpP, FpM, FuP, FuM, GpP, GpM, GvP, GvM, HpP, HpM, HwP, HwM;
+    flux_axis_cell(g, i, j, k, io, 1, 1, rho, c, FpP, FpM, FuP, FuM);
+    flux_axis_cell(g, i, j, k, io, 2, g.Nx, rho, c, GpP, GpM, GvP, GvM);
+    flux_axis_cell(g, i, j, k, io, 3, g.nxy, rho, c, HpP, HpM, HwP, HwM);
+    flux_cell_update<IO, R, STAGE>(g, i, j, k, io, pml, h, dt, FpP, FpM, FuP, FuM, GpP, GpM, GvP, GvM, HpP, HpM, HwP,
+                                   HwM);
+}
+
 // Update-mask values: 0 = not updated (outside [3,N-3)^3 or S < h), 1 = updated by the
 // grid kernel, 2 = updated by the patch kernel (reduced-precision storage only: cells
 // near point sources, kept in FP64 shadows, see fases_set_sources).
@@ -146,7 +159,8 @@ k_flux_simple(Grid g, GridIO<S, R> io, const uint8_t *__restrict__ upd, PmlSet p
     flux_cell<GridIO<S, R>, R, STAGE>(g, i, j, k, io, pml, rho, c, h, dt);
 }
 
-// The same stage update in FP64 arithmetic and FP64 shadow storage on the patch cells.
+// The same stage update in FP64 arithmetic and FP64 shadow storage on the patch cells:
+// one thread per cell (large patches) ...
 template <typename S, int STAGE>
 __global__ void __launch_bounds__(64)
 k_flux_patch(Grid g, PatchIO<S> io, const int *__restrict__ cells, PmlSet pml, double rho, double c, double h,
@@ -156,6 +170,35 @@ k_flux_patch(Grid g, PatchIO<S> io, const int *__restrict__ cells, PmlSet pml, d
     const int C = cells[t];
     const int i = C % g.Nx, j = (C / g.Nx) % g.Ny, k = (int)(C / g.nxy);
     flux_cell<PatchIO<S>, double, STAGE>(g, i, j, k, io, pml, rho, c, h, dt);
+}
+// ... or in two launches (small patches: one source's default patch is 25^3 cells, a grid
+// too small to hide the latency of one thread per cell): one thread per (cell, axis)
+// forms that axis' four face fluxes into fx[(a * np + t) * 4 ..], then one thread per cell
+// applies the stage update. The same functions in the same order as flux_cell, so the
+// same bits either way.
+template <typename S>
+__global__ void __launch_bounds__(64)
+k_flux_patch_axis(Grid g, PatchIO<S> io, const int *__restrict__ cells, double rho, double c, double *__restrict__ fx) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * io.np) return;
+    const int a = t / io.np, q = t - a * io.np;
+    const int C = cells[q];
+    const int i = C % g.Nx, j = (C / g.Nx) % g.Ny, k = (int)(C / g.nxy);
+    const long long st = a == 0 ? 1LL : (a == 1 ? (long long)g.Nx : g.nxy);
+    double *o = fx + 4LL * t;
+    flux_axis_cell(g, i, j, k, io, a + 1, st, rho, c, o[0], o[1], o[2], o[3]);
+}
+template <typename S, int STAGE>
+__global__ void __launch_bounds__(64)
+k_flux_patch_update(Grid g, PatchIO<S> io, const int *__restrict__ cells, PmlSet pml, double h, double dt,
+                    const double *__restrict__ fx) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= io.np) return;
+    const int C = cells[t];
+    const int i = C % g.Nx, j = (C / g.Nx) % g.Ny, k = (int)(C / g.nxy);
+    const double *x = fx + 4LL * t, *y = fx + 4LL * (io.np + t), *z = fx + 4LL * (2LL * io.np + t);
+    flux_cell_update<PatchIO<S>, double, STAGE>(g, i, j, k, io, pml, h, dt, x[0], x[1], x[2], x[3], y[0], y[1], y[2],
+                                                y[3], z[0], z[1], z[2], z[3]);
 }
 
 // Copies the stored fields of the patch cells into an FP64 shadow set.
