@@ -42,6 +42,23 @@ namespace {
 template <typename R> constexpr int kAdjRowsOf = std::is_same<R, double>::value ? 8 : 16;
 constexpr int kAdjTileZ = 32;
 
+// z planes per block of the tiled flux adjoint. A block holds one CU (16 waves at 94
+// VGPRs), so a room whose tiles at the default depth number fewer than the CUs leaves CUs
+// idle for the whole launch; the depth is then cut so that the tile count comes as close
+// to the CU count as it can without exceeding it (each block then marches fewer planes,
+// plus its four pipeline-fill planes): on the 192x160x128 room 3 x 14 x 4 tiles of 32
+// planes become 3 x 14 x 6 of 22.
+int adjoint_tile_z(const fases_ctx *ctx, int nbx, int nby) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->prm.device) != hipSuccess || ncu < 1)
+        return kAdjTileZ;
+    const int nz = ctx->Nz - 6;
+    const long long cols = (long long)nbx * nby;
+    if (cols * ((nz + kAdjTileZ - 1) / kAdjTileZ) >= ncu) return kAdjTileZ;
+    const long long zb = std::max<long long>(1, ncu / cols);
+    return std::max(8, (int)((nz + zb - 1) / zb));
+}
+
 // Device buffers of the adjoint, owned by the context (freed with it).
 struct AdjBufs {
     std::vector<void *> allocs;
@@ -506,9 +523,10 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
     const R kp = (R)(0.5 * P.rho * P.c * P.c), kc = (R)(0.5 * P.c), kr = (R)(0.5 / P.rho);
     // flux adjoint: tiled kernel unless the per-cell one was asked for
     const bool tiled = P.adjoint_variant != 1;
-    const int atz = kAdjTileZ;
-    dim3 agrid((ctx->Nx - 6 + adjtile::TX - 1) / adjtile::TX, (ctx->Ny - 6 + kAdjRowsOf<R> - 5) / (kAdjRowsOf<R> - 4),
-               (ctx->Nz - 6 + atz - 1) / atz);
+    const int anx = (ctx->Nx - 6 + adjtile::TX - 1) / adjtile::TX;
+    const int any = (ctx->Ny - 6 + kAdjRowsOf<R> - 5) / (kAdjRowsOf<R> - 4);
+    const int atz = adjoint_tile_z(ctx, anx, any);
+    dim3 agrid(anx, any, (ctx->Nz - 6 + atz - 1) / atz);
     // variant 2: the same tiles in the XCD-aware order, on a 1-D grid
     const int anbx = P.adjoint_variant == 2 ? (int)agrid.x : 0, anby = (int)agrid.y;
     if (anbx) agrid = dim3(agrid.x * agrid.y * agrid.z);

@@ -291,16 +291,19 @@ k_flux_adj_patch(Grid g, const R *__restrict__ L0, const R *__restrict__ L1, con
     }
 }
 
-// Tiled flux-stage adjoint. Block = 64 lanes x NW waves; lane l holds column
-// X0 - 2 + l and wave w row Y0 - 2 + w, marching over planes in z. Every cell's two
-// WENO pair VJPs per axis are evaluated once; their five window contributions reach
-// their cells through lane shuffles (x), an LDS exchange (y) and a ring of five plane
-// accumulators (z). Outputs: the inner 60 columns x (NW - 4) rows, plane k - 2 once the
-// pipeline has passed plane k + 2... i.e. two planes behind the march.
+// Tiled flux-stage adjoint. Block = 64 lanes x NW waves; lane l holds column X0 + l and
+// wave w row Y0 - 2 + w, marching over planes in z. Every cell's two WENO pair VJPs per
+// axis are evaluated once; their five window contributions reach their cells through lane
+// shuffles (x), an LDS exchange (y) and a ring of five plane accumulators (z). Outputs:
+// all 64 columns x the inner NW - 4 rows, plane k - 2 once the pipeline has passed plane
+// k + 2, i.e. two planes behind the march. The x-pair VJPs of the four cells beyond the
+// tile's x edges (X0-2, X0-1, X0+64, X0+65) that reach its outputs are evaluated by the
+// four y-halo waves (which have no x or z work) and handed to lanes 0, 1, 62, 63 through
+// LDS, in the same place of the same sum.
 namespace adjtile {
-constexpr int TX = 60;       // output columns per block
-constexpr int VX = 68;       // V tile columns: X0-4 .. X0+63
-constexpr int LXA = 66;      // adjoint tile columns: X0-3 .. X0+62
+constexpr int TX = 64;       // output columns per block
+constexpr int VX = 72;       // V tile columns: X0-4 .. X0+67
+constexpr int LXA = 70;      // adjoint tile columns: X0-3 .. X0+66
 }  // namespace adjtile
 
 template <typename R, int NW> struct AdjLds {
@@ -311,6 +314,10 @@ template <typename R, int NW> struct AdjLds {
     R lu[NW][adjtile::LXA];
     R lv[NW + 2][64];
     R gy[2][5][NW][64];          // y-pair contributions of every row: p, v
+    // x-edge contributions per output row (p, u): gp4(X0-2), gp3(X0-1), gp4(X0-1),
+    // gp0(X0+64), gp1(X0+64), gp0(X0+65)
+    R xe[2][NW][6];
+    R spare[64];                 // target of the halo slots that carry no item
 };
 
 template <typename S, typename R, bool ACC_IN, bool ACC_BASE, bool HAS_BASE, int NW>
@@ -336,59 +343,74 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
     const int X0 = 3 + TX * bx, Y0 = 3 + (NW - 4) * by, k0 = 3 + tz * bz;
     const int kend = min(k0 + tz, Nz - 3);
     const long long nxy = (long long)Nx * Ny;
-    const int x = X0 - 2 + lx, y = Y0 - 2 + wy;
+    const int x = X0 + lx, y = Y0 - 2 + wy;
     const int xc = max(0, min(x, Nx - 1)), yc = max(0, min(y, Ny - 1));
     const long long col = xc + (long long)Nx * yc;
     const bool out_row = (wy >= 2 && wy <= NW - 3) && y < Ny - 3;
-    const bool out_cell = out_row && lx >= 2 && lx <= 61 && x < Nx - 3;
+    const bool out_cell = out_row && x < Nx - 3;
+    const bool halo_wave = wy < 2 || wy > NW - 3;
     const R bdh = K.bdh;
     auto plane = [&](int k) -> long long { return nxy * (long long)max(0, min(k, Nz - 1)); };
     auto lt = [&](const R *a, long long q) -> R { return upd[q] ? bdh * a[q] : R(0); };
 
-    // halo items of this thread: items tid, tid + 64 NW, ... (see the tile shapes above)
-    constexpr int NITEM = 4 * VX + 4 * NW + 4 * NW + 4 * 64 + 2 * LXA + 2 * NW + 2 * NW + 2 * 64;
-    constexpr int NH = (NITEM + 64 * NW - 1) / (64 * NW);
-    int h_kind[NH], h_r[NH], h_c[NH];
-    long long h_off[NH];
+    // halo items of this thread: items tid, tid + 64 NW, ... (see the tile shapes above):
+    // vp rows 0, 1, NW+2, NW+3 and the four side columns of each end of the own rows,
+    // vu the same side columns, vv rows 0, 1, NW+2, NW+3; lp rows 0, NW+1 and three side
+    // columns of each end of the own rows, lu the same side columns, lv rows 0, NW+1.
+    // Each kind's items start on a multiple of 64, so the kind of a slot is uniform over a
+    // wave (kind 6: no item); a slot keeps only its LDS target and its offset in a plane, and
+    // a padding slot of a kind loads cell 0 of the plane into the spare row.
+    constexpr int n0 = 4 * VX + 8 * NW, n1 = 8 * NW, n2 = 4 * 64, n3 = 2 * LXA + 6 * NW, n4 = 6 * NW, n5 = 2 * 64;
+    constexpr int e0 = (n0 + 63) / 64 * 64, e1 = e0 + (n1 + 63) / 64 * 64, e2 = e1 + n2,
+                  e3 = e2 + (n3 + 63) / 64 * 64, e4 = e3 + (n4 + 63) / 64 * 64, e5 = e4 + n5;
+    constexpr int NH = (e5 + 64 * NW - 1) / (64 * NW);
+    int h_kind[NH];             // wave-uniform
+    R *h_dst[NH];
+    unsigned h_off[NH];         // element offset of the item's cell within a plane
 #pragma unroll
     for (int hq = 0; hq < NH; ++hq) {
-        int t = tid + hq * 64 * NW;
-        int kind = -1, r = 0, c = 0;
-        const int n0 = 4 * VX + 4 * NW, n1 = 4 * NW, n2 = 4 * 64, n3 = 2 * LXA + 2 * NW, n4 = 2 * NW, n5 = 2 * 64;
+        const int t0 = tid + hq * 64 * NW;
+        const int tw = __builtin_amdgcn_readfirstlane(t0);
+        const int kind = tw < e0 ? 0 : tw < e1 ? 1 : tw < e2 ? 2 : tw < e3 ? 3 : tw < e4 ? 4 : tw < e5 ? 5 : 6;
         const int rA[4] = {0, 1, NW + 2, NW + 3};
-        const int cA[4] = {0, 1, VX - 2, VX - 1};
-        if (t < n0) {
-            kind = 0;
+        const int cA[8] = {0, 1, 2, 3, VX - 4, VX - 3, VX - 2, VX - 1};
+        const int cL[6] = {0, 1, 2, LXA - 3, LXA - 2, LXA - 1};
+        int r = -1, c = 0, t;
+        switch (kind) {
+        case 0:
+            t = t0;
             if (t < 4 * VX) { r = rA[t / VX]; c = t % VX; }
-            else { t -= 4 * VX; c = cA[t / NW]; r = 2 + t % NW; }
-        } else if ((t -= n0) < n1) {
-            kind = 1; c = cA[t / NW]; r = t % NW;
-        } else if ((t -= n1) < n2) {
-            kind = 2; r = rA[t / 64]; c = t % 64;
-        } else if ((t -= n2) < n3) {
-            kind = 3;
+            else if ((t -= 4 * VX) < 8 * NW) { c = cA[t / NW]; r = 2 + t % NW; }
+            break;
+        case 1: t = t0 - e0; if (t < n1) { c = cA[t / NW]; r = t % NW; } break;
+        case 2: t = t0 - e1; r = rA[t / 64]; c = t % 64; break;
+        case 3:
+            t = t0 - e2;
             if (t < 2 * LXA) { r = (t / LXA) ? NW + 1 : 0; c = t % LXA; }
-            else { t -= 2 * LXA; c = (t / NW) ? LXA - 1 : 0; r = 1 + t % NW; }
-        } else if ((t -= n3) < n4) {
-            kind = 4; c = (t / NW) ? LXA - 1 : 0; r = t % NW;
-        } else if ((t -= n4) < n5) {
-            kind = 5; r = (t / 64) ? NW + 1 : 0; c = t % 64;
+            else if ((t -= 2 * LXA) < 6 * NW) { c = cL[t / NW]; r = 1 + t % NW; }
+            break;
+        case 4: t = t0 - e3; if (t < n4) { c = cL[t / NW]; r = t % NW; } break;
+        case 5: t = t0 - e4; r = (t / 64) ? NW + 1 : 0; c = t % 64; break;
+        default: break;
         }
-        long long off = 0;
-        if (kind >= 0) {
+        h_kind[hq] = kind;
+        h_dst[hq] = &T.spare[lx];
+        h_off[hq] = 0;
+        if (r >= 0) {
             int gx, gy;
             if (kind <= 2) {            // V tiles: vp (r, c) -> (Y0-4+r, X0-4+c); vu rows are wy; vv cols are lanes
-                gx = (kind == 2) ? X0 - 2 + c : X0 - 4 + c;
+                gx = (kind == 2) ? X0 + c : X0 - 4 + c;
                 gy = (kind == 1) ? Y0 - 2 + r : Y0 - 4 + r;
             } else {                    // adjoint tiles: lp (r, c) -> (Y0-3+r, X0-3+c)
-                gx = (kind == 5) ? X0 - 2 + c : X0 - 3 + c;
+                gx = (kind == 5) ? X0 + c : X0 - 3 + c;
                 gy = (kind == 4) ? Y0 - 2 + r : Y0 - 3 + r;
             }
             gx = max(0, min(gx, Nx - 1));
             gy = max(0, min(gy, Ny - 1));
-            off = gx + (long long)Nx * gy;
+            h_off[hq] = (unsigned)(gx + Nx * gy);
+            h_dst[hq] = kind == 0 ? &T.vp[r][c] : kind == 1 ? &T.vu[r][c] : kind == 2 ? &T.vv[r][c]
+                      : kind == 3 ? &T.lp[r][c] : kind == 4 ? &T.lu[r][c] : &T.lv[r][c];
         }
-        h_kind[hq] = kind; h_r[hq] = r; h_c[hq] = c; h_off[hq] = off;
     }
 
     // Plane-relative access of the own column (buffer descriptors: scalar plane base,
@@ -443,37 +465,36 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
         }
         if (xy) {
             const long long pk = nxy * (long long)k;
-            T.vp[wy + 2][lx + 2] = zp[Z2];
-            T.vu[wy][lx + 2] = vld(V.u, k, sc.s[1]);
+            T.vp[wy + 2][lx + 4] = zp[Z2];
+            T.vu[wy][lx + 4] = vld(V.u, k, sc.s[1]);
             T.vv[wy + 2][lx] = vld(V.v, k, sc.s[2]);
-            T.lp[wy + 1][lx + 1] = ap[Z1];
-            T.lu[wy][lx + 1] = ltp(L1, pk);
+            T.lp[wy + 1][lx + 3] = ap[Z1];
+            T.lu[wy][lx + 3] = ltp(L1, pk);
             T.lv[wy + 1][lx] = ltp(L2, pk);
 #pragma unroll
             for (int hq = 0; hq < NH; ++hq) {
-                if (h_kind[hq] < 0) continue;
                 const long long q = pk + h_off[hq];
-                const int r = h_r[hq], c = h_c[hq];
-                switch (h_kind[hq]) {
-                case 0: T.vp[r][c] = CV::ld(V.p[q], sc.s[0]); break;
-                case 1: T.vu[r][c] = CV::ld(V.u[q], sc.s[1]); break;
-                case 2: T.vv[r][c] = CV::ld(V.v[q], sc.s[2]); break;
-                case 3: T.lp[r][c] = lt(L0, q); break;
-                case 4: T.lu[r][c] = lt(L1, q); break;
-                default: T.lv[r][c] = lt(L2, q); break;
+                switch (h_kind[hq]) {             // wave-uniform
+                case 0: *h_dst[hq] = CV::ld(V.p[q], sc.s[0]); break;
+                case 1: *h_dst[hq] = CV::ld(V.u[q], sc.s[1]); break;
+                case 2: *h_dst[hq] = CV::ld(V.v[q], sc.s[2]); break;
+                case 3: *h_dst[hq] = lt(L0, q); break;
+                case 4: *h_dst[hq] = lt(L1, q); break;
+                case 5: *h_dst[hq] = lt(L2, q); break;
+                default: break;
                 }
             }
             __syncthreads();
             // ---- y pairs of every row
             {
-                const R a_m = T.lp[wy][lx + 1], a_c = T.lp[wy + 1][lx + 1], a_p = T.lp[wy + 2][lx + 1];
+                const R a_m = T.lp[wy][lx + 3], a_c = T.lp[wy + 1][lx + 3], a_p = T.lp[wy + 2][lx + 3];
                 const R b_m = T.lv[wy][lx], b_c = T.lv[wy + 1][lx], b_p = T.lv[wy + 2][lx];
                 R mpL, mpR, mvL, mvR, t0, t1, t2, t3;
                 godunov_vjp(a_p - a_c, b_p - b_c, K.kp, K.kc, K.kr, mpL, t0, mvL, t1);
                 godunov_vjp(a_c - a_m, b_c - b_m, K.kp, K.kc, K.kr, t2, mpR, t3, mvR);
                 R gp[5], gv[5];
-                weno_pair_vjp(T.vp[wy][lx + 2], T.vp[wy + 1][lx + 2], T.vp[wy + 2][lx + 2], T.vp[wy + 3][lx + 2],
-                              T.vp[wy + 4][lx + 2], mpL, mpR, gp);
+                weno_pair_vjp(T.vp[wy][lx + 4], T.vp[wy + 1][lx + 4], T.vp[wy + 2][lx + 4], T.vp[wy + 3][lx + 4],
+                              T.vp[wy + 4][lx + 4], mpL, mpR, gp);
                 weno_pair_vjp(T.vv[wy][lx], T.vv[wy + 1][lx], T.vv[wy + 2][lx], T.vv[wy + 3][lx], T.vv[wy + 4][lx],
                               mvL, mvR, gv);
 #pragma unroll
@@ -482,26 +503,69 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
                     T.gy[1][q][wy][lx] = gv[q];
                 }
             }
-            // ---- x pairs of the output rows (all lanes of those waves)
-            R cxp = 0, cxu = 0;
+            // ---- x pairs of the output rows (all lanes of those waves); the four y-halo
+            // waves evaluate the x pairs of the cells beyond the tile's x edges
+            R xp[5] = {0, 0, 0, 0, 0}, xu[5] = {0, 0, 0, 0, 0};   // own g[2], then the neighbours' terms
             if (out_row) {
-                const R a_m = T.lp[wy + 1][lx], a_c = T.lp[wy + 1][lx + 1], a_p = T.lp[wy + 1][lx + 2];
-                const R b_m = T.lu[wy][lx], b_c = T.lu[wy][lx + 1], b_p = T.lu[wy][lx + 2];
+                const R a_m = T.lp[wy + 1][lx + 2], a_c = T.lp[wy + 1][lx + 3], a_p = T.lp[wy + 1][lx + 4];
+                const R b_m = T.lu[wy][lx + 2], b_c = T.lu[wy][lx + 3], b_p = T.lu[wy][lx + 4];
                 R mpL, mpR, muL, muR, t0, t1, t2, t3;
                 godunov_vjp(a_p - a_c, b_p - b_c, K.kp, K.kc, K.kr, mpL, t0, muL, t1);
                 godunov_vjp(a_c - a_m, b_c - b_m, K.kp, K.kc, K.kr, t2, mpR, t3, muR);
                 R gp[5], gu[5];
-                weno_pair_vjp(T.vp[wy + 2][lx], T.vp[wy + 2][lx + 1], T.vp[wy + 2][lx + 2], T.vp[wy + 2][lx + 3],
-                              T.vp[wy + 2][lx + 4], mpL, mpR, gp);
-                weno_pair_vjp(T.vu[wy][lx], T.vu[wy][lx + 1], T.vu[wy][lx + 2], T.vu[wy][lx + 3], T.vu[wy][lx + 4],
+                weno_pair_vjp(T.vp[wy + 2][lx + 2], T.vp[wy + 2][lx + 3], T.vp[wy + 2][lx + 4], T.vp[wy + 2][lx + 5],
+                              T.vp[wy + 2][lx + 6], mpL, mpR, gp);
+                weno_pair_vjp(T.vu[wy][lx + 2], T.vu[wy][lx + 3], T.vu[wy][lx + 4], T.vu[wy][lx + 5], T.vu[wy][lx + 6],
                               muL, muR, gu);
-                cxp = gp[2] + __shfl_up(gp[3], 1, 64) + __shfl_up(gp[4], 2, 64) + __shfl_down(gp[1], 1, 64) +
-                      __shfl_down(gp[0], 2, 64);
-                cxu = gu[2] + __shfl_up(gu[3], 1, 64) + __shfl_up(gu[4], 2, 64) + __shfl_down(gu[1], 1, 64) +
-                      __shfl_down(gu[0], 2, 64);
+                xp[0] = gp[2];
+                xp[1] = __shfl_up(gp[3], 1, 64);
+                xp[2] = __shfl_up(gp[4], 2, 64);
+                xp[3] = __shfl_down(gp[1], 1, 64);
+                xp[4] = __shfl_down(gp[0], 2, 64);
+                xu[0] = gu[2];
+                xu[1] = __shfl_up(gu[3], 1, 64);
+                xu[2] = __shfl_up(gu[4], 2, 64);
+                xu[3] = __shfl_down(gu[1], 1, 64);
+                xu[4] = __shfl_down(gu[0], 2, 64);
+            } else if (halo_wave) {
+                constexpr int NO = NW - 4;                       // output rows
+                const int h = wy < 2 ? wy : wy - (NW - 4);      // 0..3
+                if (lx < 2 * NO) {
+                    // item t: var (p, u), edge cell e (X0-2, X0-1, X0+64, X0+65), output row r
+                    const int t = h * 2 * NO + lx;
+                    const int var = t / (4 * NO), e = (t % (4 * NO)) / NO, r = t % NO;
+                    const int dx = e < 2 ? e - 2 : 62 + e;
+                    const int w = r + 2;                         // the output row's wave
+                    const R *lpr = &T.lp[w + 1][dx + 2], *lur = &T.lu[w][dx + 2];
+                    R mpL, mpR, muL, muR, t0, t1, t2, t3;
+                    godunov_vjp(lpr[2] - lpr[1], lur[2] - lur[1], K.kp, K.kc, K.kr, mpL, t0, muL, t1);
+                    godunov_vjp(lpr[1] - lpr[0], lur[1] - lur[0], K.kp, K.kc, K.kr, t2, mpR, t3, muR);
+                    const R *vw = var ? &T.vu[w][dx + 2] : &T.vp[w + 2][dx + 2];
+                    R g[5];
+                    weno_pair_vjp(vw[0], vw[1], vw[2], vw[3], vw[4], var ? muL : mpL, var ? muR : mpR, g);
+                    R *o = T.xe[var][w];
+                    if (e == 0) {
+                        o[0] = g[4];
+                    } else if (e == 1) {
+                        o[1] = g[3];
+                        o[2] = g[4];
+                    } else if (e == 2) {
+                        o[3] = g[0];
+                        o[4] = g[1];
+                    } else {
+                        o[5] = g[0];
+                    }
+                }
             }
             __syncthreads();
             if (out_row) {
+                // the tile edges' terms from the halo waves, then the same sum as inside
+                if (lx == 0) { xp[1] = T.xe[0][wy][1]; xp[2] = T.xe[0][wy][0]; xu[1] = T.xe[1][wy][1]; xu[2] = T.xe[1][wy][0]; }
+                if (lx == 1) { xp[2] = T.xe[0][wy][2]; xu[2] = T.xe[1][wy][2]; }
+                if (lx == 62) { xp[4] = T.xe[0][wy][3]; xu[4] = T.xe[1][wy][3]; }
+                if (lx == 63) { xp[3] = T.xe[0][wy][4]; xp[4] = T.xe[0][wy][5]; xu[3] = T.xe[1][wy][4]; xu[4] = T.xe[1][wy][5]; }
+                const R cxp = xp[0] + xp[1] + xp[2] + xp[3] + xp[4];
+                const R cxu = xu[0] + xu[1] + xu[2] + xu[3] + xu[4];
                 R cyp = 0, cyv = 0;
 #pragma unroll
                 for (int q = 0; q < 5; ++q) {
