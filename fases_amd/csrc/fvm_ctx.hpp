@@ -335,13 +335,17 @@ int launch_tiled_nw(fases_ctx *ctx, int stage, const FieldSet<S> &fi, const Fiel
 // keeps the fastest. Sets 1 and 2 are rewritten by the next step before they are read and
 // neither stage advances the PML states, so tuning changes no state; no candidate changes
 // a result. A fixed tile_cols / tile_rows / tile_z restricts the search to that value.
+// Depths: 16, 32, 64, 128 and, on a room whose 16-deep tiles do not fill the resident
+// block slots of the chip (three 8-row blocks or one 16-row block per CU), the depth at
+// which the tile count comes closest to the slot count without exceeding it.
 template <typename S, typename R>
 int autotune_tile(fases_ctx *ctx) {
     const int fix_tx = ctx->prm.tile_cols, fix_nw = ctx->prm.tile_rows, fix_tz = ctx->prm.tile_z;
     if (fix_tz > 0 && fix_nw > 0 && fix_tx > 0) { ctx->tz = fix_tz; ctx->nw = fix_nw; ctx->tx = fix_tx; return FASES_OK; }
     const fases_params &P = ctx->prm;
     const int nz = ctx->Nz - 6;
-    const int cand[4] = {16, 32, 64, 128};
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, P.device) != hipSuccess) ncu = 0;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -351,6 +355,14 @@ int autotune_tile(fases_ctx *ctx) {
         if (fix_tx > 0 && tx != fix_tx) continue;
         for (int nw : {16, 8}) {
             if (fix_nw > 0 && nw != fix_nw) continue;
+            std::vector<int> cand{16, 32, 64, 128};
+            const long long cols = (long long)((ctx->Nx - 6 + tx - 1) / tx) * ((ctx->Ny - 6 + nw - 1) / nw);
+            const long long slots = (long long)ncu * (nw == 8 ? 3 : 1);
+            if (cols * ((nz + 15) / 16) < slots) {
+                const long long zb = std::max<long long>(1, slots / cols);
+                const int tf = std::max(4, (int)((nz + zb - 1) / zb));
+                if (tf < 16) cand.insert(cand.begin(), tf);
+            }
             for (int tz : cand) {
                 if (fix_tz > 0 && tz != fix_tz) continue;
                 if (fix_tz <= 0 && tz > 16 && (tz / 2) >= nz) break;   // deeper blocks than the grid add nothing
