@@ -154,24 +154,25 @@ int zero_ade(fases_ctx *ctx, const AdeState &x) {
 // and the corner weight wx wy wz, grouped by target.
 int build_probe_csr(fases_ctx *ctx, AdjBufs &A) {
     const int nB = ctx->nB;
-    std::vector<int> base((size_t)nB), mat((size_t)nB);
-    std::vector<double> geo((size_t)nB * kGeoEntries);
-    HIPCHK(hipMemcpy(base.data(), ctx->d_bbase, (size_t)nB * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(mat.data(), ctx->d_bmat, (size_t)nB * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(geo.data(), ctx->d_bgeo, geo.size() * 8, hipMemcpyDeviceToHost));
+    const std::vector<int> &mat = ctx->h_bmat;
     struct E { int tgt, b; double w; };
     std::vector<E> es;
     es.reserve((size_t)nB * 8);
     const long long nx = ctx->Nx, nxy = (long long)ctx->Nx * ctx->Ny;
     for (int b = 0; b < nB; ++b) {
-        if (mat[(size_t)b] & 0x100) { es.push_back({base[(size_t)b], b, 1.0}); continue; }
-        const double sx = geo[(size_t)kGeoSx * nB + b], sy = geo[(size_t)kGeoSy * nB + b],
-                     sz = geo[(size_t)kGeoSz * nB + b];
+        // the probe geometry the boundary kernels compute for this cell
+        const int C = ctx->h_bcell[(size_t)b];
+        const int i = (int)(C % nx), j = (int)((C / nx) % ctx->Ny), k = (int)(C / nxy);
+        const ProbeGeo q = probe_geo(i, j, k, ctx->Nx, ctx->Ny, ctx->Nz, (double)ctx->h_bdist[(size_t)b],
+                                     ctx->h_bnrm[(size_t)b], ctx->h_bnrm[(size_t)nB + b],
+                                     ctx->h_bnrm[2 * (size_t)nB + b], ctx->prm.h, ctx->prm.tol);
+        if (mat[(size_t)b] & 0x100) { es.push_back({q.base, b, 1.0}); continue; }
+        const double sx = q.sx, sy = q.sy, sz = q.sz;
         for (int dz = 0; dz < 2; ++dz)
             for (int dy = 0; dy < 2; ++dy)
                 for (int dx = 0; dx < 2; ++dx) {
                     const double w = (dx ? sx : 1.0 - sx) * (dy ? sy : 1.0 - sy) * (dz ? sz : 1.0 - sz);
-                    es.push_back({(int)(base[(size_t)b] + dx + nx * dy + nxy * dz), b, w});
+                    es.push_back({(int)(q.base + dx + nx * dy + nxy * dz), b, w});
                 }
     }
     std::stable_sort(es.begin(), es.end(), [](const E &a, const E &b) { return a.tgt < b.tgt; });
@@ -485,7 +486,7 @@ int backward_impl(fases_ctx *ctx, const double *dJ, double *grad_mat, double *gr
     };
     const Ade ade{ctx->K, ctx->L, ctx->d_Rinf, ctx->d_A, ctx->d_zeta, ctx->d_B, ctx->d_C, ctx->d_alpha, ctx->d_beta,
                   ctx->d_Keff, ctx->d_Leff};
-    const Bnd<double> bd{ctx->nB, ctx->d_bcell, ctx->d_bbase, ctx->d_bmat, ctx->d_bgeo};
+    const Bnd bd = bnd_of(ctx);
     const double As = 6.0 * P.h * P.h;
     const double coef0 = P.dt * P.c * P.c * P.rho * As / (P.h * P.h * P.h);
     int rc;

@@ -662,14 +662,15 @@ __device__ inline double probe_wout(const FieldSet<S> &V, const Grid &g, int bas
 
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_ade_replay(Grid g, FieldSet<S> V0, FieldSet<S> V1, Bnd<double> bd, Ade ade, double rho, double c, double dt,
+k_ade_replay(Grid g, FieldSet<S> V0, FieldSet<S> V1, Bnd bd, Ade ade, double rho, double c, double dt,
              AdeState x0, AdeState x1, AdeState x2, Scl<double> sc) {
 #pragma clang fp contract(off)
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= bd.n) return;
-    const int base = bd.base[b], mw = bd.mat[b], m = mw & 0xFF;
-    const double sx = bd.at(kGeoSx, b), sy = bd.at(kGeoSy, b), sz = bd.at(kGeoSz, b);
-    const double nx = bd.at(kGeoNx, b), ny = bd.at(kGeoNy, b), nz = bd.at(kGeoNz, b);
+    const BndCell bc = bnd_cell(g, bd, b);
+    const int base = bc.q.base, mw = bc.mw, m = mw & 0xFF;
+    const double sx = bc.q.sx, sy = bc.q.sy, sz = bc.q.sz;
+    const double nx = bc.nx, ny = bc.ny, nz = bc.nz;
     const double Z0 = rho * c;
     const double w0 = probe_wout(V0, g, base, mw, sx, sy, sz, nx, ny, nz, Z0, sc);
     const double w1 = probe_wout(V1, g, base, mw, sx, sy, sz, nx, ny, nz, Z0, sc);
@@ -712,18 +713,18 @@ k_ade_replay(Grid g, FieldSet<S> V0, FieldSet<S> V1, Bnd<double> bd, Ade ade, do
 // is coalesced). The stage coefficients come as a, b and 1/c.
 template <typename S, typename R, bool SET_MID, bool SET_R>
 __global__ void __launch_bounds__(256)
-k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd<double> bd, Ade ade, double rho, double c, double dt, double as,
+k_boundary_adj(Grid g, FieldSet<S> Vin, Bnd bd, Ade ade, double rho, double c, double dt, double as,
                double bs, double ic, AdeState xmid, AdeState xr, AdeState lnew, AdeState lmid, AdeState lr,
                R *L0, R *L1, R *L2, R *L3, double *__restrict__ lD, double *__restrict__ G, int P,
                Scl<double> sc) {
 #pragma clang fp contract(off)
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= bd.n) return;
-    const int C = bd.cell[b], base = bd.base[b], mw = bd.mat[b], m = mw & 0xFF;
-    const double nx = bd.at(kGeoNx, b), ny = bd.at(kGeoNy, b), nz = bd.at(kGeoNz, b), chi = bd.at(kGeoChi, b);
+    const BndCell cel = bnd_cell(g, bd, b);
+    const int C = cel.C, base = cel.q.base, mw = cel.mw, m = mw & 0xFF;
+    const double nx = cel.nx, ny = cel.ny, nz = cel.nz, chi = cel.q.chi;
     const double Z0 = rho * c;
-    const double wout = probe_wout(Vin, g, base, mw, bd.at(kGeoSx, b), bd.at(kGeoSy, b), bd.at(kGeoSz, b), nx, ny,
-                                   nz, Z0, sc);
+    const double wout = probe_wout(Vin, g, base, mw, cel.q.sx, cel.q.sy, cel.q.sz, nx, ny, nz, Z0, sc);
     const int K = ade.K, L = ade.L;
     const long long nB = bd.n;
     double *Gb = G + b;                                 // entry p at Gb[p nB]

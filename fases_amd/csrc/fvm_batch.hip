@@ -135,7 +135,7 @@ __global__ void k_sources_batch(const SrcJob<S> *__restrict__ jobs, const int *_
 template <typename S> struct BndJob {
     Grid g;
     S *U[3][4];
-    Bnd<double> bd;
+    Bnd bd;
     Ade ade;
     double *phi[3], *psi1[3], *psi2[3];
     double rho, c, dt;
@@ -164,18 +164,18 @@ k_boundary_batch(const BndJob<S> *__restrict__ jobs, const int *__restrict__ off
     if constexpr (std::is_same<S, double>::value) {
         const AdeBufs ab{J.phi[pl], J.phi[pm], J.phi[pr], J.psi1[pl], J.psi1[pm], J.psi1[pr],
                          J.psi2[pl], J.psi2[pm], J.psi2[pr]};
-        boundary_cell<S, double>(t - off[j], J.g, fi, fo, J.bd, J.ade, J.rho, J.c, J.dt, a, b, cc, ab, J.sc);
+        boundary_cell<S>(t - off[j], J.g, fi, fo, J.bd, J.ade, J.rho, J.c, J.dt, a, b, cc, ab, J.sc);
     } else {
         // the single-context loop's stage-collapsed ADE: x0 = phi[mid], x3 -> phi[logical 0]
         const int p0 = phys(0, parity);
         const AdeRkBufs rb{J.phi[pm], J.psi1[pm], J.psi2[pm], J.phi[p0], J.psi1[p0], J.psi2[p0], J.aux};
         const double Z0 = J.rho * J.c, iz = 1.0 / (2.0 * Z0);
         if (stage == 0)
-            boundary_cell_rk<S, double, 0>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
+            boundary_cell_rk<S, 0>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
         else if (stage == 1)
-            boundary_cell_rk<S, double, 1>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
+            boundary_cell_rk<S, 1>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
         else
-            boundary_cell_rk<S, double, 2>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
+            boundary_cell_rk<S, 2>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
     }
 }
 
@@ -298,7 +298,7 @@ int build_phase(Batch &bt, const std::vector<fases_ctx *> &jobs, const std::vect
             BndJob<S> b{};
             b.g = g;
             std::memcpy(b.U, f.U, sizeof b.U);
-            b.bd = Bnd<double>{c->nB, c->d_bcell, c->d_bbase, c->d_bmat, c->d_bgeo};
+            b.bd = bnd_of(c);
             b.ade = Ade{c->K, c->L, c->d_Rinf, c->d_A, c->d_zeta, c->d_B, c->d_C, c->d_alpha, c->d_beta,
                         c->d_Keff, c->d_Leff};
             for (int k = 0; k < 3; ++k) { b.phi[k] = c->phi[k]; b.psi1[k] = c->psi1[k]; b.psi2[k] = c->psi2[k]; }

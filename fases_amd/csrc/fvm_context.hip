@@ -331,52 +331,47 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
     ctx->h_patch.clear();
     for (auto &sh : ctx->shadow) sh = nullptr;
     // boundary list in bmap order: cells in [3, N-3)^3 with oid in [1, 254]
-    // (functions3D.cuh:466-479, filterABC.h:95-108) and their fixed geometry.
-    std::vector<int> bcell, bbase, bmat;
-    std::vector<double> bgeo;
+    // (functions3D.cuh:466-479, filterABC.h:95-108): cell, material word, signed distance
+    // and normal; the probe geometry follows from these (probe_geo, on the device per
+    // stage and here once for the centred-probe flag).
+    std::vector<int> bcell, bmat;
+    std::vector<float> bdist;
+    std::vector<double> bnrm;
     const double h = P.h, tol = P.tol;
     for (long long C = 0; C < N; ++C) {
         if (oid[C] == 0 || oid[C] == 0xFF) continue;
         const int i = (int)(C % Nx), j = (int)((C / Nx) % Ny), k = (int)(C / nxy);
         if (i < NG || i >= Nx - NG || j < NG || j >= Ny - NG || k < NG || k >= Nz - NG) continue;
         const double s = S[C];
-        // normal n = -grad S/|grad S| (central differences), foot point W = C + s n,
-        // probe D = W - (s + 2h) n, trilinear base corner of D (clamped to the grid).
+        // normal n = -grad S/|grad S| (central differences)
         const double gx = ((double)S[C + 1] - (double)S[C - 1]) * (0.5 / h);
         const double gy = ((double)S[C + Nx] - (double)S[C - Nx]) * (0.5 / h);
         const double gz = ((double)S[C + nxy] - (double)S[C - nxy]) * (0.5 / h);
         double gn = std::sqrt(gx * gx + gy * gy + gz * gz);
         gn = std::fmax(gn, 1e-12);
         const double nx = -gx / gn, ny = -gy / gn, nz = -gz / gn;
-        const double Cx = (i + 0.5) * h, Cy = (j + 0.5) * h, Cz = (k + 0.5) * h;
-        const double Wx = Cx + s * nx, Wy = Cy + s * ny, Wz = Cz + s * nz;
-        const double l = s + 2.0 * h;
-        const double Dx = Wx - l * nx, Dy = Wy - l * ny, Dz = Wz - l * nz;
-        const double qx = Dx / h - 0.5, qy = Dy / h - 0.5, qz = Dz / h - 0.5;
-        int bi = (int)std::floor(qx), bj = (int)std::floor(qy), bk = (int)std::floor(qz);
-        bi = std::max(0, std::min(bi, Nx - 2));
-        bj = std::max(0, std::min(bj, Ny - 2));
-        bk = std::max(0, std::min(bk, Nz - 2));
+        const ProbeGeo q = probe_geo(i, j, k, Nx, Ny, Nz, s, nx, ny, nz, h, tol);
         bcell.push_back((int)C);
-        bbase.push_back(bi + Nx * (bj + Ny * bk));
-        const double fx = qx - (double)bi, fy = qy - (double)bj, fz = qz - (double)bk;
-        const int centred = (fx == 0.0 && fy == 0.0 && fz == 0.0) ? 0x100 : 0;
+        const int centred = (q.sx == 0.0 && q.sy == 0.0 && q.sz == 0.0) ? 0x100 : 0;
         bmat.push_back(((int)oid[C] - 1) | centred);
-        for (double v : {fx, fy, fz, nx, ny, nz, chi_band(s, h, tol)}) bgeo.push_back(v);
+        bdist.push_back(S[C]);
+        for (double v : {nx, ny, nz}) bnrm.push_back(v);
     }
     ctx->nB = (int)bcell.size();
-    {   // cell-major records -> entry-major [kGeoEntries][nB]
-        std::vector<double> t(bgeo.size());
+    {   // cell-major normals -> entry-major [3][nB]
+        std::vector<double> t(bnrm.size());
         for (size_t b = 0; b < bcell.size(); ++b)
-            for (int f = 0; f < kGeoEntries; ++f) t[(size_t)f * bcell.size() + b] = bgeo[b * kGeoEntries + f];
-        bgeo.swap(t);
+            for (int a = 0; a < 3; ++a) t[(size_t)a * bcell.size() + b] = bnrm[b * 3 + a];
+        bnrm.swap(t);
     }
     if ((rc = upload(ctx, &ctx->d_bcell, bcell.data(), bcell.size()))) return rc;
-    if ((rc = upload(ctx, &ctx->d_bbase, bbase.data(), bbase.size()))) return rc;
     if ((rc = upload(ctx, &ctx->d_bmat, bmat.data(), bmat.size()))) return rc;
-    if ((rc = upload(ctx, &ctx->d_bgeo, bgeo.data(), bgeo.size()))) return rc;
+    if ((rc = upload(ctx, &ctx->d_bdist, bdist.data(), bdist.size()))) return rc;
+    if ((rc = upload(ctx, &ctx->d_bnrm, bnrm.data(), bnrm.size()))) return rc;
     ctx->h_bcell = bcell;
     ctx->h_bmat = bmat;
+    ctx->h_bdist = bdist;
+    ctx->h_bnrm = bnrm;
     ctx->h_oid.assign(oid, oid + N);
     // PML tables and psi tiles (main3D.cu:804-828)
     const float *key = P.pml_keys;   // U D R L F B

@@ -52,9 +52,12 @@ struct fases_ctx {
     std::vector<int> h_patch;
     // boundary list
     int nB = 0;
-    int *d_bcell = nullptr, *d_bbase = nullptr, *d_bmat = nullptr;
-    double *d_bgeo = nullptr;
+    int *d_bcell = nullptr, *d_bmat = nullptr;
+    float *d_bdist = nullptr;            // signed distance of each boundary cell
+    double *d_bnrm = nullptr;            // unit normals, [3][nB]
     std::vector<int> h_bcell, h_bmat;
+    std::vector<float> h_bdist;
+    std::vector<double> h_bnrm;
     // materials and ADE states
     int nMat = 0, K = 1, L = 1;
     double *d_Rinf = nullptr, *d_A = nullptr, *d_zeta = nullptr, *d_B = nullptr, *d_C = nullptr,
@@ -450,6 +453,11 @@ bool rescale_due(long long n, int period) {
 // chip, and the split adds the face-flux round trip).
 constexpr int kPatchSplitBelow = 65536;
 
+// The boundary list of a context as the boundary kernels take it.
+inline Bnd bnd_of(const fases_ctx *ctx) {
+    return Bnd{ctx->nB, ctx->d_bcell, ctx->d_bmat, ctx->d_bdist, ctx->d_bnrm, ctx->prm.h, ctx->prm.tol};
+}
+
 // Stage-collapsed ADE parameters and buffers of a context (reduced-precision storage):
 // x0 = phi[1] (the step's start state), x3 -> phi[0].
 inline AdeRk ade_rk_of(const fases_ctx *ctx) {
@@ -480,8 +488,7 @@ int launch_stages(fases_ctx *ctx, long long n, const FieldSet<S> sets[4], double
             ctx->d_Keff, ctx->d_Leff};
     // Geometry stays FP64 for every field precision: a blend weight or normal rounded
     // once to FP32 is a bias that every step repeats (measured: FP32-room IR error x3.5).
-    typedef double G;
-    Bnd<G> bd{ctx->nB, ctx->d_bcell, ctx->d_bbase, ctx->d_bmat, ctx->d_bgeo};
+    const Bnd bd = bnd_of(ctx);
     // (field set in, set out, RK weight, source sample, ADE a,b,c, ADE buffers left/mid/right)
     struct StageCfg { int in, out; double w; long long nn; double a, b, c; int l, mid, r; };
     const StageCfg cfg[3] = {
@@ -552,7 +559,7 @@ int launch_stages(fases_ctx *ctx, long long n, const FieldSet<S> sets[4], double
                                ctx->src_opt == FASES_SRC_SINE ? ctx->n_samp : 0LL, ctx->n_src, ctx->src_opt, q.nn, q.w, P.dt, P.c, P.rho, As, P.h,
                                ctx->n_patch > 0 ? ctx->d_pidx : nullptr, sh[q.out], sc.s[0], sc.is[0]);
         if (ctx->nB > 0 && std::is_same<S, double>::value)
-            hipLaunchKernelGGL((k_boundary<S, G>), dim3((ctx->nB + sb - 1) / sb), dim3(sb), 0, ctx->stream, g,
+            hipLaunchKernelGGL((k_boundary<S>), dim3((ctx->nB + sb - 1) / sb), dim3(sb), 0, ctx->stream, g,
                                sets[q.in], sets[q.out], bd, ade, P.rho, P.c, P.dt, q.a, q.b, q.c,
                                AdeBufs{ctx->phi[q.l], ctx->phi[q.mid], ctx->phi[q.r], ctx->psi1[q.l],
                                        ctx->psi1[q.mid], ctx->psi1[q.r], ctx->psi2[q.l], ctx->psi2[q.mid],
@@ -566,13 +573,13 @@ int launch_stages(fases_ctx *ctx, long long n, const FieldSet<S> sets[4], double
             const double Z0 = P.rho * P.c, iz = 1.0 / (2.0 * Z0);
             const dim3 bg((unsigned)((ctx->nB + sb - 1) / sb));
             if (s == 0)
-                hipLaunchKernelGGL((k_boundary_rk<S, G, 0>), bg, dim3(sb), 0, ctx->stream, g, sets[q.in],
+                hipLaunchKernelGGL((k_boundary_rk<S, 0>), bg, dim3(sb), 0, ctx->stream, g, sets[q.in],
                                    sets[q.out], bd, rk, Z0, iz, rb, scd);
             else if (s == 1)
-                hipLaunchKernelGGL((k_boundary_rk<S, G, 1>), bg, dim3(sb), 0, ctx->stream, g, sets[q.in],
+                hipLaunchKernelGGL((k_boundary_rk<S, 1>), bg, dim3(sb), 0, ctx->stream, g, sets[q.in],
                                    sets[q.out], bd, rk, Z0, iz, rb, scd);
             else
-                hipLaunchKernelGGL((k_boundary_rk<S, G, 2>), bg, dim3(sb), 0, ctx->stream, g, sets[q.in],
+                hipLaunchKernelGGL((k_boundary_rk<S, 2>), bg, dim3(sb), 0, ctx->stream, g, sets[q.in],
                                    sets[q.out], bd, rk, Z0, iz, rb, scd);
         }
     }

@@ -113,6 +113,7 @@ __device__ inline void godunov(R pL, R pR, R uL, R uR, R rho, R c, R &Fp, R &Fu)
 // Immersed-boundary blend weight: 1 for s <= h, smoothstep down to 0 at s = tol.
 template <typename R>
 __host__ __device__ inline R chi_band(R s, R h, R tol) {
+#pragma clang fp contract(off)
     if (s >= tol) return R(0);
     if (s <= h) return R(1);
     const R r = (s - h) / (tol - h);

@@ -288,18 +288,67 @@ __global__ void k_sources(S *__restrict__ p, const int *__restrict__ cell, const
 // q = (1-chi) q + chi q_b acts on the stage OUTPUT. All arithmetic is FP64.
 // ADE stage (a,b,c): x_left = (a x_mid + b (x_right + dt xdot(x_right))) / c.
 // ------------------------------------------------------------------------------------
-template <typename G> struct Bnd {
+// Probe geometry of boundary cell (i, j, k) from its signed distance s (the SDF's float
+// value) and unit normal n (functions3D.cuh:466-479, filterABC.h:95-108): foot point
+// W = C + s n, probe D = W - (s + 2h) n, the trilinear base corner of D clamped to the
+// grid, the fractions of D inside that corner's cell, and the blend weight chi. The host
+// builds the boundary list with this function and the boundary kernels recompute it per
+// cell from (s, n): both sides round alike (no contraction, IEEE division and floor), so
+// the kernels see the host's bits while the list holds 36 bytes per cell instead of 68.
+struct ProbeGeo {
+    int base;               // grid index of the trilinear base corner
+    double sx, sy, sz;      // fractions of the probe point inside the base cell
+    double chi;             // blend weight
+};
+__host__ __device__ inline ProbeGeo probe_geo(int i, int j, int k, int Nx, int Ny, int Nz, double s, double nx,
+                                              double ny, double nz, double h, double tol) {
+#pragma clang fp contract(off)
+    const double Cx = (i + 0.5) * h, Cy = (j + 0.5) * h, Cz = (k + 0.5) * h;
+    const double Wx = Cx + s * nx, Wy = Cy + s * ny, Wz = Cz + s * nz;
+    const double l = s + 2.0 * h;
+    const double Dx = Wx - l * nx, Dy = Wy - l * ny, Dz = Wz - l * nz;
+    const double qx = Dx / h - 0.5, qy = Dy / h - 0.5, qz = Dz / h - 0.5;
+    int bi = (int)floor(qx), bj = (int)floor(qy), bk = (int)floor(qz);
+    bi = bi < 0 ? 0 : (bi > Nx - 2 ? Nx - 2 : bi);
+    bj = bj < 0 ? 0 : (bj > Ny - 2 ? Ny - 2 : bj);
+    bk = bk < 0 ? 0 : (bk > Nz - 2 ? Nz - 2 : bk);
+    ProbeGeo r;
+    r.base = bi + Nx * (bj + Ny * bk);
+    r.sx = qx - (double)bi;
+    r.sy = qy - (double)bj;
+    r.sz = qz - (double)bk;
+    r.chi = chi_band(s, h, tol);
+    return r;
+}
+
+struct Bnd {
     int n;
     const int *cell;        // grid index of the boundary cell
-    const int *base;        // grid index of the trilinear base corner of the probe point
     const int *mat;         // bits 0-7: material index (oid - 1); bit 8: probe on a cell centre
-    const G *geo;           // [7][n] (entry-major): sx, sy, sz, nx, ny, nz, chi
-    // entry f of boundary cell b: consecutive cells are consecutive addresses
-    __device__ inline G at(int f, int b) const { return geo[(long long)f * n + b]; }
+    const float *dist;      // signed distance S of the cell
+    const double *nrm;      // [3][n] (entry-major): unit normal nx, ny, nz
+    double h, tol;          // cell size; SDF distance where the blend weight reaches 0
 };
 
-// Boundary geometry entries of Bnd::geo.
-enum BndGeo { kGeoSx = 0, kGeoSy, kGeoSz, kGeoNx, kGeoNy, kGeoNz, kGeoChi, kGeoEntries };
+// What the boundary kernels use of boundary cell b: its grid cell, material word, normal
+// and probe geometry (consecutive b are consecutive addresses in every array).
+struct BndCell {
+    int C, mw;
+    double nx, ny, nz;
+    ProbeGeo q;
+};
+__device__ inline BndCell bnd_cell(const Grid &g, const Bnd &bd, int b) {
+    BndCell r;
+    r.C = bd.cell[b];
+    r.mw = bd.mat[b];
+    r.nx = bd.nrm[b];
+    r.ny = bd.nrm[(long long)bd.n + b];
+    r.nz = bd.nrm[2LL * bd.n + b];
+    const unsigned C = (unsigned)r.C, nxy = (unsigned)g.nxy, nx = (unsigned)g.Nx;
+    const unsigned k = C / nxy, rem = C - k * nxy, j = rem / nx, i = rem - j * nx;
+    r.q = probe_geo((int)i, (int)j, (int)k, g.Nx, g.Ny, g.Nz, (double)bd.dist[b], r.nx, r.ny, r.nz, bd.h, bd.tol);
+    return r;
+}
 
 // ADE state k of boundary cell b in an [K][nB] state buffer (pole-major, so the lanes of a
 // wave, consecutive boundary cells, touch consecutive addresses).
@@ -343,9 +392,9 @@ struct AdeBufs {
     const double *p2_m, *p2_r;
 };
 
-template <typename S, typename G>
+template <typename S>
 __device__ inline void boundary_cell(int b, const Grid &g, const FieldSet<S> &in, const FieldSet<S> &out,
-                                     const Bnd<G> &bd, const Ade &ade, double rho, double c, double dt, double as,
+                                     const Bnd &bd, const Ade &ade, double rho, double c, double dt, double as,
                                      double bs, double cs, const AdeBufs &ab, const Scl<double> &sc) {
 #pragma clang fp contract(off)
     double *__restrict__ phi_l = ab.phi_l, *__restrict__ p1_l = ab.p1_l, *__restrict__ p2_l = ab.p2_l;
@@ -354,9 +403,10 @@ __device__ inline void boundary_cell(int b, const Grid &g, const FieldSet<S> &in
     const double *__restrict__ p2_m = ab.p2_m, *__restrict__ p2_r = ab.p2_r;
     typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type R;
     typedef Conv<S, R> CV;
-    const int C = bd.cell[b], base = bd.base[b], mw = bd.mat[b];
+    const BndCell bc = bnd_cell(g, bd, b);
+    const int C = bc.C, base = bc.q.base, mw = bc.mw;
     const int m = mw & 0xFF;
-    const double nx = bd.at(kGeoNx, b), ny = bd.at(kGeoNy, b), nz = bd.at(kGeoNz, b), chi = bd.at(kGeoChi, b);
+    const double nx = bc.nx, ny = bc.ny, nz = bc.nz, chi = bc.q.chi;
     double pD, uD, vD, wD;
     if (mw & 0x100) {
         // probe exactly on a cell centre (all trilinear fractions 0): the interpolant is
@@ -366,7 +416,7 @@ __device__ inline void boundary_cell(int b, const Grid &g, const FieldSet<S> &in
         vD = (double)CV::ld(in.v[base], (R)sc.s[2]);
         wD = (double)CV::ld(in.w[base], (R)sc.s[3]);
     } else {
-        const double sx = bd.at(kGeoSx, b), sy = bd.at(kGeoSy, b), sz = bd.at(kGeoSz, b);
+        const double sx = bc.q.sx, sy = bc.q.sy, sz = bc.q.sz;
         pD = tri_probe(in.p, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[0]);
         uD = tri_probe(in.u, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[1]);
         vD = tri_probe(in.v, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[2]);
@@ -419,13 +469,13 @@ __device__ inline void boundary_cell(int b, const Grid &g, const FieldSet<S> &in
     out.p[C] = CV::st((R)((1.0 - chi) * p_o + chi * pb), (R)sc.is[0]);
 }
 
-template <typename S, typename G>
+template <typename S>
 __global__ void __launch_bounds__(256)
-k_boundary(Grid g, FieldSet<S> in, FieldSet<S> out, Bnd<G> bd, Ade ade, double rho, double c, double dt,
+k_boundary(Grid g, FieldSet<S> in, FieldSet<S> out, Bnd bd, Ade ade, double rho, double c, double dt,
            double as, double bs, double cs, AdeBufs ab, Scl<double> sc) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= bd.n) return;
-    boundary_cell<S, G>(b, g, in, out, bd, ade, rho, c, dt, as, bs, cs, ab, sc);
+    boundary_cell<S>(b, g, in, out, bd, ade, rho, c, dt, as, bs, cs, ab, sc);
 }
 
 // ------------------------------------------------------------------------------------
@@ -466,15 +516,16 @@ struct AdeRkBufs {
     double *aux;
 };
 
-template <typename S, typename G, int STAGE>
+template <typename S, int STAGE>
 __device__ inline void boundary_cell_rk(int b, const Grid &g, const FieldSet<S> &in, const FieldSet<S> &out,
-                                        const Bnd<G> &bd, const AdeRk &ade, double Z0, double inv_2z0,
+                                        const Bnd &bd, const AdeRk &ade, double Z0, double inv_2z0,
                                         const AdeRkBufs &ab, const Scl<double> &sc) {
     typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type R;
     typedef Conv<S, R> CV;
-    const int C = bd.cell[b], base = bd.base[b], mw = bd.mat[b];
+    const BndCell bc = bnd_cell(g, bd, b);
+    const int C = bc.C, base = bc.q.base, mw = bc.mw;
     const int m = mw & 0xFF;
-    const double nx = bd.at(kGeoNx, b), ny = bd.at(kGeoNy, b), nz = bd.at(kGeoNz, b), chi = bd.at(kGeoChi, b);
+    const double nx = bc.nx, ny = bc.ny, nz = bc.nz, chi = bc.q.chi;
     double pD, uD, vD, wD;
     if (mw & 0x100) {
         pD = (double)CV::ld(in.p[base], (R)sc.s[0]);
@@ -482,7 +533,7 @@ __device__ inline void boundary_cell_rk(int b, const Grid &g, const FieldSet<S> 
         vD = (double)CV::ld(in.v[base], (R)sc.s[2]);
         wD = (double)CV::ld(in.w[base], (R)sc.s[3]);
     } else {
-        const double sx = bd.at(kGeoSx, b), sy = bd.at(kGeoSy, b), sz = bd.at(kGeoSz, b);
+        const double sx = bc.q.sx, sy = bc.q.sy, sz = bc.q.sz;
         pD = tri_probe(in.p, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[0]);
         uD = tri_probe(in.u, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[1]);
         vD = tri_probe(in.v, base, 0, sx, sy, sz, g.Nx, g.nxy, sc.s[2]);
@@ -552,13 +603,13 @@ __device__ inline void boundary_cell_rk(int b, const Grid &g, const FieldSet<S> 
     out.p[C] = CV::st((R)((1.0 - chi) * p_o + chi * pb), (R)sc.is[0]);
 }
 
-template <typename S, typename G, int STAGE>
+template <typename S, int STAGE>
 __global__ void __launch_bounds__(256)
-k_boundary_rk(Grid g, FieldSet<S> in, FieldSet<S> out, Bnd<G> bd, AdeRk ade, double Z0, double inv_2z0,
+k_boundary_rk(Grid g, FieldSet<S> in, FieldSet<S> out, Bnd bd, AdeRk ade, double Z0, double inv_2z0,
               AdeRkBufs ab, Scl<double> sc) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= bd.n) return;
-    boundary_cell_rk<S, G, STAGE>(b, g, in, out, bd, ade, Z0, inv_2z0, ab, sc);
+    boundary_cell_rk<S, STAGE>(b, g, in, out, bd, ade, Z0, inv_2z0, ab, sc);
 }
 
 // Receiver gather after a step: ir[r*stride + n] = p(cell_r), vel[(r*stride + n)*3 + a].
