@@ -21,10 +21,16 @@ namespace {
 __host__ __device__ inline int phys(int L, int p) { return L < 2 ? (L ^ p) : 2; }
 
 // Job index of global item t given ascending offsets off[0..nj-1] (off[0] = 0).
+// (the last j with off[j] <= t, so jobs without items are skipped), by bisection: a
+// handful of dependent loads at the start of a block or thread instead of up to nj.
 __device__ inline int job_of(const int *__restrict__ off, int nj, int t) {
-    int j = 0;
-    while (j + 1 < nj && t >= off[j + 1]) ++j;
-    return j;
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 template <typename S> struct FluxJob {
@@ -40,16 +46,17 @@ template <typename S> struct FluxJob {
 // Rows (waves) per block of the batched flux kernel: 8-wave blocks, two per CU, overlap
 // each other's barriers (the faster shape on the 512x384x256 room and on the batch rooms).
 constexpr int kBatchRows = 8;
-// Tile width of the batched flux kernel: 64 columns (the faster width on rooms of up to
-// 256 cells across; results do not depend on the width).
-constexpr int kBatchTX = 64;
+// Tile width of a job in the batched flux kernel: the width its own autotuned run uses
+// (62 or 64 columns; results do not depend on the width). Jobs of each width share one
+// launch per stage.
+constexpr int kBatchWidths[2] = {62, 64};
 
 // PF: the cPML fused in the tile (some job of the phase has a band). Without it the tile
 // is held to 80 VGPRs, three blocks per CU, like the single-context kernel.
-template <typename S, int STAGE, bool PF>
+template <typename S, int STAGE, bool PF, int TX>
 __global__ void __launch_bounds__(64 * kBatchRows) __attribute__((amdgpu_waves_per_eu(PF ? 1 : 6, 8)))
 k_flux_batch(const FluxJob<S> *__restrict__ jobs, const int *__restrict__ boff, int nj, int parity) {
-    __shared__ TileLds<float, kBatchRows, kBatchTX> L;
+    __shared__ TileLds<float, kBatchRows, TX> L;
     const int j = job_of(boff, nj, blockIdx.x);
     const FluxJob<S> &J = jobs[j];
     const int lb = blockIdx.x - boff[j];
@@ -60,8 +67,7 @@ k_flux_batch(const FluxJob<S> *__restrict__ jobs, const int *__restrict__ boff, 
                           J.U[pb][2], J.U[pb][3], J.U[po][0], J.U[po][1], J.U[po][2], J.U[po][3]};
     const TileK<float> K = J.K;
     const Scl<float> sc = J.sc;
-    flux_tile<S, float, STAGE, kBatchRows, kBatchTX, PF>(L, J.Nx, J.Ny, J.Nz, F, J.upd, &J.pml, K, J.dt, sc, J.tz, bx, by,
-                                                         bz);
+    flux_tile<S, float, STAGE, kBatchRows, TX, PF>(L, J.Nx, J.Ny, J.Nz, F, J.upd, &J.pml, K, J.dt, sc, J.tz, bx, by, bz);
 }
 
 template <typename S> struct PatchJob {
@@ -75,14 +81,21 @@ template <typename S> struct PatchJob {
     Scl<double> sc;
 };
 
+// Each job's items start on a wave boundary in the patch and boundary tables (offsets
+// padded to 64), so a wave's job is wave-uniform: its descriptor comes through scalar loads
+// and stays in scalar registers.
+constexpr int kWaveItems = 64;
+inline int pad_wave(int n) { return (n + kWaveItems - 1) / kWaveItems * kWaveItems; }
+
 template <typename S, int STAGE>
 __global__ void __launch_bounds__(64)
 k_patch_batch(const PatchJob<S> *__restrict__ jobs, const int *__restrict__ off, int nj, int parity) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= off[nj]) return;
-    const int j = job_of(off, nj, t);
+    const int j = __builtin_amdgcn_readfirstlane(job_of(off, nj, t));
     const PatchJob<S> &J = jobs[j];
     const int lt = t - off[j];
+    if (lt >= J.np) return;
     constexpr int in = STAGE == 0 ? 0 : (STAGE == 1 ? 1 : 2), out = STAGE == 1 ? 2 : 1;
     const int pi = phys(in, parity), pb = phys(0, parity), po = phys(out, parity);
     auto fs = [&](int q) { return FieldSet<S>{J.U[q][0], J.U[q][1], J.U[q][2], J.U[q][3]}; };
@@ -144,19 +157,20 @@ template <typename S> struct BndJob {
     double *aux;
 };
 
-template <typename S>
+template <typename S, int STAGE>
 __global__ void __launch_bounds__(256)
-k_boundary_batch(const BndJob<S> *__restrict__ jobs, const int *__restrict__ off, int nj, int stage, int parity) {
+k_boundary_batch(const BndJob<S> *__restrict__ jobs, const int *__restrict__ off, int nj, int parity) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= off[nj]) return;
-    const int j = job_of(off, nj, t);
+    const int j = __builtin_amdgcn_readfirstlane(job_of(off, nj, t));
     const BndJob<S> &J = jobs[j];
+    if (t - off[j] >= J.bd.n) return;
     // (set in, set out, ADE a, b, c, buffers left / mid / right) per stage (main3D.cu:1290-1306)
-    const int in = stage, out = stage == 1 ? 2 : 1;
-    const double a = stage == 0 ? 0.0 : (stage == 1 ? 3.0 : 1.0);
-    const double b = stage == 2 ? 2.0 : 1.0;
-    const double cc = stage == 0 ? 1.0 : (stage == 1 ? 4.0 : 3.0);
-    const int l = stage == 1 ? 2 : 0, r = stage == 0 ? 1 : (stage == 1 ? 0 : 2);
+    const int in = STAGE, out = STAGE == 1 ? 2 : 1;
+    const double a = STAGE == 0 ? 0.0 : (STAGE == 1 ? 3.0 : 1.0);
+    const double b = STAGE == 2 ? 2.0 : 1.0;
+    const double cc = STAGE == 0 ? 1.0 : (STAGE == 1 ? 4.0 : 3.0);
+    const int l = STAGE == 1 ? 2 : 0, r = STAGE == 0 ? 1 : (STAGE == 1 ? 0 : 2);
     const int pi = phys(in, parity), po = phys(out, parity);
     const int pl = phys(l, parity), pm = phys(1, parity), pr = phys(r, parity);
     const FieldSet<S> fi{J.U[pi][0], J.U[pi][1], J.U[pi][2], J.U[pi][3]};
@@ -170,12 +184,7 @@ k_boundary_batch(const BndJob<S> *__restrict__ jobs, const int *__restrict__ off
         const int p0 = phys(0, parity);
         const AdeRkBufs rb{J.phi[pm], J.psi1[pm], J.psi2[pm], J.phi[p0], J.psi1[p0], J.psi2[p0], J.aux};
         const double Z0 = J.rho * J.c, iz = 1.0 / (2.0 * Z0);
-        if (stage == 0)
-            boundary_cell_rk<S, 0>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
-        else if (stage == 1)
-            boundary_cell_rk<S, 1>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
-        else
-            boundary_cell_rk<S, 2>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
+        boundary_cell_rk<S, STAGE>(t - off[j], J.g, fi, fo, J.bd, J.rk, Z0, iz, rb, J.sc);
     }
 }
 
@@ -206,14 +215,17 @@ __global__ void k_step_advance(long long *d_n) {
 
 // Device tables of one phase (the jobs still running).
 template <typename S> struct Phase {
-    int nj = 0, nblk = 0, npatch = 0, nsrc = 0, nbnd = 0, nrec = 0;
+    int npatch = 0, nsrc = 0, nbnd = 0, nrec = 0;
     bool any_pml = false;                // some job has a cPML band: the fused tile
-    FluxJob<S> *flux = nullptr;
+    // flux jobs per tile width (kBatchWidths): job tables, block offsets, counts
+    int fnj[2] = {0, 0}, fnblk[2] = {0, 0};
+    FluxJob<S> *fjobs[2] = {nullptr, nullptr};
+    int *fboff[2] = {nullptr, nullptr};
     PatchJob<S> *patch = nullptr;
     SrcJob<S> *src = nullptr;
     BndJob<S> *bnd = nullptr;
     RecJob<S> *rec = nullptr;
-    int *boff = nullptr, *poff = nullptr, *soff = nullptr, *doff = nullptr, *roff = nullptr;
+    int *poff = nullptr, *soff = nullptr, *doff = nullptr, *roff = nullptr;
 };
 
 struct Batch {
@@ -249,12 +261,12 @@ struct Batch {
 
 template <typename S>
 int build_phase(Batch &bt, const std::vector<fases_ctx *> &jobs, const std::vector<long long> &nsteps, Phase<S> &ph) {
-    std::vector<FluxJob<S>> fj;
+    std::vector<FluxJob<S>> fj[2];
     std::vector<PatchJob<S>> pj;
     std::vector<SrcJob<S>> sj;
     std::vector<BndJob<S>> bj;
     std::vector<RecJob<S>> rj;
-    std::vector<int> boff{0}, poff{0}, soff{0}, doff{0}, roff{0};
+    std::vector<int> boff[2] = {{0}, {0}}, poff{0}, soff{0}, doff{0}, roff{0};
     for (size_t q = 0; q < jobs.size(); ++q) {
         fases_ctx *c = jobs[q];
         const fases_params &P = c->prm;
@@ -262,16 +274,17 @@ int build_phase(Batch &bt, const std::vector<fases_ctx *> &jobs, const std::vect
         const Scl<double> scd = field_scales<double>(c);
         const Grid g{c->Nx, c->Ny, c->Nz, (long long)c->Nx * c->Ny};
         FluxJob<S> f{};
+        const int w = c->tx == kBatchWidths[0] ? 0 : 1;
         f.Nx = c->Nx; f.Ny = c->Ny; f.Nz = c->Nz; f.tz = c->tz;
-        f.nbx = (c->Nx - 6 + kBatchTX - 1) / kBatchTX;
+        f.nbx = (c->Nx - 6 + kBatchWidths[w] - 1) / kBatchWidths[w];
         f.nby = (c->Ny - 6 + kBatchRows - 1) / kBatchRows;
         const int nbz = (c->Nz - 6 + c->tz - 1) / c->tz;
         for (int s = 0; s < 3; ++s)
             for (int v = 0; v < 4; ++v) f.U[s][v] = (S *)c->U[s][v];
         f.upd = c->d_upd; f.pml = c->pml; f.K = tile_constants<float>(P); f.dt = (float)P.dt; f.sc = scf;
         ph.any_pml = ph.any_pml || c->pml.on;
-        fj.push_back(f);
-        boff.push_back(boff.back() + f.nbx * f.nby * nbz);
+        fj[w].push_back(f);
+        boff[w].push_back(boff[w].back() + f.nbx * f.nby * nbz);
         if (c->n_patch > 0) {
             PatchJob<S> p{};
             p.g = g;
@@ -280,7 +293,7 @@ int build_phase(Batch &bt, const std::vector<fases_ctx *> &jobs, const std::vect
             p.pidx = c->d_pidx; p.cells = c->d_patch; p.np = c->n_patch; p.pml = c->pml;
             p.rho = P.rho; p.c = P.c; p.h = P.h; p.dt = P.dt; p.sc = scd;
             pj.push_back(p);
-            poff.push_back(poff.back() + c->n_patch);
+            poff.push_back(poff.back() + pad_wave(c->n_patch));
         }
         if (c->n_src > 0) {
             SrcJob<S> s{};
@@ -306,7 +319,7 @@ int build_phase(Batch &bt, const std::vector<fases_ctx *> &jobs, const std::vect
             b.rk = ade_rk_of(c);
             b.aux = c->d_ade_aux;
             bj.push_back(b);
-            doff.push_back(doff.back() + c->nB);
+            doff.push_back(doff.back() + pad_wave(c->nB));
         }
         if (c->n_rec > 0) {
             RecJob<S> r{};
@@ -316,17 +329,36 @@ int build_phase(Batch &bt, const std::vector<fases_ctx *> &jobs, const std::vect
             roff.push_back(roff.back() + c->n_rec);
         }
     }
-    ph.nj = (int)fj.size();
-    ph.nblk = boff.back(); ph.npatch = poff.back(); ph.nsrc = soff.back(); ph.nbnd = doff.back(); ph.nrec = roff.back();
+    ph.npatch = poff.back(); ph.nsrc = soff.back(); ph.nbnd = doff.back(); ph.nrec = roff.back();
     int rc;
-    if ((rc = bt.put(fj, &ph.flux)) || (rc = bt.put(boff, &ph.boff))) return rc;
+    for (int w = 0; w < 2; ++w) {
+        ph.fnj[w] = (int)fj[w].size();
+        ph.fnblk[w] = boff[w].back();
+        if ((rc = bt.put(fj[w], &ph.fjobs[w])) || (rc = bt.put(boff[w], &ph.fboff[w]))) return rc;
+    }
     if ((rc = bt.put(pj, &ph.patch)) || (rc = bt.put(poff, &ph.poff))) return rc;
     if ((rc = bt.put(sj, &ph.src)) || (rc = bt.put(soff, &ph.soff))) return rc;
     if ((rc = bt.put(bj, &ph.bnd)) || (rc = bt.put(doff, &ph.doff))) return rc;
     if ((rc = bt.put(rj, &ph.rec)) || (rc = bt.put(roff, &ph.roff))) return rc;
-    // table sizes per kind (number of jobs that contribute)
-    ph.nj = (int)fj.size();
     return FASES_OK;
+}
+
+// The flux stage s of the phase's jobs of tile width TX (table w).
+template <typename S, int TX>
+void launch_flux_batch(hipStream_t st, const Phase<S> &ph, int w, int s, int parity) {
+    const dim3 g(ph.fnblk[w]), b(64, kBatchRows);
+    const FluxJob<S> *J = ph.fjobs[w];
+    const int *off = ph.fboff[w];
+    const int nj = ph.fnj[w];
+    if (ph.any_pml) {
+        if (s == 0) hipLaunchKernelGGL((k_flux_batch<S, 0, true, TX>), g, b, 0, st, J, off, nj, parity);
+        else if (s == 1) hipLaunchKernelGGL((k_flux_batch<S, 1, true, TX>), g, b, 0, st, J, off, nj, parity);
+        else hipLaunchKernelGGL((k_flux_batch<S, 2, true, TX>), g, b, 0, st, J, off, nj, parity);
+    } else {
+        if (s == 0) hipLaunchKernelGGL((k_flux_batch<S, 0, false, TX>), g, b, 0, st, J, off, nj, parity);
+        else if (s == 1) hipLaunchKernelGGL((k_flux_batch<S, 1, false, TX>), g, b, 0, st, J, off, nj, parity);
+        else hipLaunchKernelGGL((k_flux_batch<S, 2, false, TX>), g, b, 0, st, J, off, nj, parity);
+    }
 }
 
 // Kernel sequence of one step of a phase at the given parity.
@@ -334,18 +366,8 @@ template <typename S>
 void launch_batch_step(hipStream_t st, const Phase<S> &ph, int np_jobs, int ns_jobs, int nb_jobs, int nr_jobs,
                        int parity, long long *d_n) {
     for (int s = 0; s < 3; ++s) {
-        if (ph.nblk > 0) {
-            const dim3 g(ph.nblk), b(64, kBatchRows);
-            if (ph.any_pml) {
-                if (s == 0) hipLaunchKernelGGL((k_flux_batch<S, 0, true>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
-                else if (s == 1) hipLaunchKernelGGL((k_flux_batch<S, 1, true>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
-                else hipLaunchKernelGGL((k_flux_batch<S, 2, true>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
-            } else {
-                if (s == 0) hipLaunchKernelGGL((k_flux_batch<S, 0, false>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
-                else if (s == 1) hipLaunchKernelGGL((k_flux_batch<S, 1, false>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
-                else hipLaunchKernelGGL((k_flux_batch<S, 2, false>), g, b, 0, st, ph.flux, ph.boff, ph.nj, parity);
-            }
-        }
+        if (ph.fnblk[0] > 0) launch_flux_batch<S, 62>(st, ph, 0, s, parity);
+        if (ph.fnblk[1] > 0) launch_flux_batch<S, 64>(st, ph, 1, s, parity);
         if (ph.npatch > 0) {
             const dim3 g((ph.npatch + 63) / 64);
             if (s == 0)
@@ -358,9 +380,15 @@ void launch_batch_step(hipStream_t st, const Phase<S> &ph, int np_jobs, int ns_j
         if (ph.nsrc > 0)
             hipLaunchKernelGGL((k_sources_batch<S>), dim3((ph.nsrc + 63) / 64), dim3(64), 0, st, ph.src, ph.soff,
                                ns_jobs, s, parity, (const long long *)d_n);
-        if (ph.nbnd > 0)
-            hipLaunchKernelGGL((k_boundary_batch<S>), dim3((ph.nbnd + 255) / 256), dim3(256), 0, st, ph.bnd, ph.doff,
-                               nb_jobs, s, parity);
+        if (ph.nbnd > 0) {
+            const dim3 g((ph.nbnd + 255) / 256);
+            if (s == 0)
+                hipLaunchKernelGGL((k_boundary_batch<S, 0>), g, dim3(256), 0, st, ph.bnd, ph.doff, nb_jobs, parity);
+            else if (s == 1)
+                hipLaunchKernelGGL((k_boundary_batch<S, 1>), g, dim3(256), 0, st, ph.bnd, ph.doff, nb_jobs, parity);
+            else
+                hipLaunchKernelGGL((k_boundary_batch<S, 2>), g, dim3(256), 0, st, ph.bnd, ph.doff, nb_jobs, parity);
+        }
     }
     if (ph.nrec > 0)
         hipLaunchKernelGGL((k_receivers_batch<S>), dim3((ph.nrec + 63) / 64), dim3(64), 0, st, ph.rec, ph.roff,
