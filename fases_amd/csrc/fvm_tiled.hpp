@@ -756,8 +756,11 @@ k_pml_band(int Nx, int Ny, TileFields<S> F, const uint8_t *__restrict__ upd, con
     if (bx) diffs(1, F.iu, sc.s[1], Lpx, Lux);
     if (by) diffs(Nx, F.iv, sc.s[2], Lpy, Lvy);
     if (bz) diffs(nxy, F.iw, sc.s[3], Lpz, Lwz);
-    R po = CV::ld(F.op[C], sc.s[0]), uo = CV::ld(F.ou[C], sc.s[1]);
-    R vo = CV::ld(F.ov[C], sc.s[2]), wo = CV::ld(F.ow[C], sc.s[3]);
+    // the velocities of the cell's band axes only (pml_apply leaves the others alone)
+    R po = CV::ld(F.op[C], sc.s[0]);
+    R uo = bx ? CV::ld(F.ou[C], sc.s[1]) : R(0);
+    R vo = by ? CV::ld(F.ov[C], sc.s[2]) : R(0);
+    R wo = bz ? CV::ld(F.ow[C], sc.s[3]) : R(0);
     const R wgt = STAGE == 0 ? R(1) : (STAGE == 1 ? R(0.25) : R(2) / R(3));
     pml_apply<R, STAGE == 2>(pm, i, j, k, Nx, Ny, po, uo, vo, wo, Lpx, Lpy, Lpz, Lux, Lvy, Lwz, wgt * dt);
     F.op[C] = CV::st(po, sc.is[0]);
