@@ -443,6 +443,8 @@ def main() -> int:
                      "kernel": "flux stage (WENO5-Z + Godunov + SSP-RK3 + cPML)",
                      "alg_bytes_per_launch": int(alg_bytes_launch),
                      "avg_launch_us": round(avg_launch_s * 1e6, 1),
+                     "avg_launch_source": "HIP events around the flux stages of every 16th timed step "
+                                          "(the stream they run on)",
                      "copy_kernel_gbs": round(copy_gbs, 1),
                      "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs > 0 else None,
                      "valu": ({"wave_instr_per_launch": valu_instr,

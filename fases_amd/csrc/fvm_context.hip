@@ -90,6 +90,8 @@ float half_value(uint16_t h) {
     return v;
 }
 
+constexpr long long kKernelEventEvery = 16;
+
 template <typename S, typename R>
 int run_impl(fases_ctx *ctx, long long nsteps, double *ir, double *vel, double *elapsed_ms, double *kernel_ms) {
     const long long need = (long long)ctx->n_rec * nsteps;
@@ -103,15 +105,21 @@ int run_impl(fases_ctx *ctx, long long nsteps, double *ir, double *vel, double *
     }
     hipEvent_t ev[2];
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    // flux-stage events on every kKernelEventEvery-th step only: an event record is a queue
+    // barrier with a cache release (~5 us on the 512x384x256 room), which on every step
+    // would lengthen the run it measures by about 1.4 %
     std::vector<hipEvent_t> kev;
-    if (kernel_ms) kev.reserve((size_t)nsteps * 6);
+    long long sampled = 0;
+    if (kernel_ms) kev.reserve((size_t)(nsteps / kKernelEventEvery + 1) * 6);
     HIPCHK(hipEventRecord(ev[0], ctx->stream));
     for (long long q = 0; q < nsteps; ++q) {
         if (std::is_same<S, __half>::value && rescale_due(ctx->step, ctx->prm.fp16_rescale_every)) {
             int rc = rescale_fields<S>(ctx);
             if (rc) return rc;
         }
-        int rc = launch_step<S, R>(ctx, ctx->step, kernel_ms ? &kev : nullptr);
+        const bool sample = kernel_ms && q % kKernelEventEvery == 0;
+        sampled += sample;
+        int rc = launch_step<S, R>(ctx, ctx->step, sample ? &kev : nullptr);
         if (rc) return rc;
         rc = launch_receivers<S>(ctx, nsteps, q);
         if (rc) return rc;
@@ -132,7 +140,7 @@ int run_impl(fases_ctx *ctx, long long nsteps, double *ir, double *vel, double *
             tot += km;
         }
         for (auto &e : kev) (void)hipEventDestroy(e);
-        *kernel_ms = tot;
+        *kernel_ms = sampled > 0 ? tot * (double)nsteps / (double)sampled : 0.0;   // scaled to all steps
     }
     if (ctx->n_rec > 0) {
         if (ir) HIPCHK(hipMemcpy(ir, ctx->d_ir, (size_t)need * sizeof(double), hipMemcpyDeviceToHost));
@@ -434,7 +442,8 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
         ctx->n_pml_cells = (int)bandc.size();
         ctx->d_pml_cells = nullptr;
         if (!bandc.empty() && (rc = upload(ctx, &ctx->d_pml_cells, bandc.data(), bandc.size()))) return rc;
-        HIPCHK(hipStreamSynchronize(ctx->stream));   // bandc is a local
+        HIPCHK(hipStreamSynchronize(ctx->stream));   // bandc is moved below
+        ctx->h_pml_cells = std::move(bandc);
     }
     ctx->n_src = 0; ctx->n_rec = 0; ctx->ir_cap = 0; ctx->step = 0;
     ctx->tz = 0;
@@ -544,6 +553,16 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
                     }
         }
         HIPCHK(hipMemcpyAsync(ctx->d_upd, ctx->h_upd.data(), ctx->h_upd.size(), hipMemcpyHostToDevice, ctx->stream));
+        if (!ctx->h_pml_cells.empty()) {
+            // the cPML pass covers the band's grid-kernel cells only: the patch update
+            // applies the cPML of its own cells in FP64
+            std::vector<int> bandc;
+            for (int c0 : ctx->h_pml_cells)
+                if (ctx->h_upd[(size_t)c0] == 1) bandc.push_back(c0);
+            if ((rc = upload_replace(ctx, &ctx->d_pml_cells, bandc.data(), bandc.size()))) return rc;
+            ctx->n_pml_cells = (int)bandc.size();
+            HIPCHK(hipStreamSynchronize(ctx->stream));   // bandc is a local
+        }
         ctx->n_patch = (int)patch.size();
         ctx->h_patch = patch;
         if (patch.empty()) {                       // no source any more: drop the old patch

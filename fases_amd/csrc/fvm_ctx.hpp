@@ -71,8 +71,9 @@ struct fases_ctx {
     // PML (device tables in pml, host copies per axis x, y, z)
     PmlSet pml{};
     PmlSet *d_pml = nullptr;      // device copy of pml (tiled path's cPML pass)
-    int *d_pml_cells = nullptr;   // updated cells inside a band, raster order
+    int *d_pml_cells = nullptr;   // grid-kernel cells (update mask 1) inside a band, raster order
     int n_pml_cells = 0;
+    std::vector<int> h_pml_cells; // the same before any source patch took cells out
     std::vector<int> h_pml_band[3];
     std::vector<double> h_pml_kc[3], h_pml_b[3], h_pml_a[3];
     // sources / receivers
@@ -303,13 +304,13 @@ int launch_tiled(fases_ctx *ctx, int stage, const FieldSet<S> &fi, const FieldSe
     if (ctx->n_pml_cells > 0) {       // cPML of the band cells on the stage output
         const dim3 pg((unsigned)((ctx->n_pml_cells + 255) / 256));
         if (stage == 0)
-            hipLaunchKernelGGL((k_pml_band<S, R, 0>), pg, dim3(256), 0, ctx->stream, ctx->Nx, ctx->Ny, F, ctx->d_upd,
+            hipLaunchKernelGGL((k_pml_band<S, R, 0>), pg, dim3(256), 0, ctx->stream, ctx->Nx, ctx->Ny, F,
                                ctx->d_pml_cells, ctx->n_pml_cells, ctx->d_pml, K, dt, sc);
         else if (stage == 1)
-            hipLaunchKernelGGL((k_pml_band<S, R, 1>), pg, dim3(256), 0, ctx->stream, ctx->Nx, ctx->Ny, F, ctx->d_upd,
+            hipLaunchKernelGGL((k_pml_band<S, R, 1>), pg, dim3(256), 0, ctx->stream, ctx->Nx, ctx->Ny, F,
                                ctx->d_pml_cells, ctx->n_pml_cells, ctx->d_pml, K, dt, sc);
         else
-            hipLaunchKernelGGL((k_pml_band<S, R, 2>), pg, dim3(256), 0, ctx->stream, ctx->Nx, ctx->Ny, F, ctx->d_upd,
+            hipLaunchKernelGGL((k_pml_band<S, R, 2>), pg, dim3(256), 0, ctx->stream, ctx->Nx, ctx->Ny, F,
                                ctx->d_pml_cells, ctx->n_pml_cells, ctx->d_pml, K, dt, sc);
     }
     return FASES_OK;

@@ -720,12 +720,11 @@ k_flux_tiled(int Nx, int Ny, int Nz, TileFields<S> F, const uint8_t *__restrict_
 // longer holds registers in every tile of the room.
 template <typename S, typename R, int STAGE>
 __global__ void __launch_bounds__(256)
-k_pml_band(int Nx, int Ny, TileFields<S> F, const uint8_t *__restrict__ upd, const int *__restrict__ cells, int n,
+k_pml_band(int Nx, int Ny, TileFields<S> F, const int *__restrict__ cells, int n,
            const PmlSet *__restrict__ pmlp, TileK<R> K, R dt, Scl<R> sc) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const long long C = cells[t];
-    if (upd[C] != 1) return;
     typedef Conv<S, R> CV;
     const PmlSet &pm = *pmlp;
     const long long nxy = (long long)Nx * Ny;

@@ -257,8 +257,9 @@ int fases_reset(fases_ctx *ctx);
 /* Advances nsteps SSP-RK3 steps from the current step counter. ir [n_rec*nsteps]
  * (row r = receiver r) receives p after each step, vel [n_rec*nsteps*3] (nullable)
  * vx,vy,vz. *elapsed_ms (nullable) = HIP-event time of the step loop alone.
- * *kernel_ms (nullable) = summed HIP-event time of the flux-stage launches (events are
- * recorded around each of them only when kernel_ms is non-NULL). */
+ * *kernel_ms (nullable) = HIP-event time of the flux-stage launches, scaled to all nsteps
+ * from the steps it is measured on: when kernel_ms is non-NULL, events are recorded around
+ * the flux stages of every 16th step (from the first), and only there. */
 int fases_run(fases_ctx *ctx, int64_t nsteps, double *ir, double *vel,
               double *elapsed_ms, double *kernel_ms);
 
