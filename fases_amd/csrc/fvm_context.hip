@@ -336,6 +336,9 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
     ctx->n_patch = 0;
     ctx->d_patch = nullptr;
     ctx->d_pidx = nullptr;
+    ctx->d_patch_fm = ctx->d_patch_fe = nullptr;      // (freed with every other buffer above)
+    ctx->d_patch_edges = ctx->d_patch_eid = nullptr;
+    ctx->n_patch_edge = 0;
     ctx->h_patch.clear();
     for (auto &sh : ctx->shadow) sh = nullptr;
     // boundary list in bmap order: cells in [3, N-3)^3 with oid in [1, 254]
@@ -568,8 +571,12 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
         if (patch.empty()) {                       // no source any more: drop the old patch
             if ((rc = dfree(ctx, ctx->d_patch))) return rc;
             ctx->d_patch = nullptr;
-            if ((rc = dfree(ctx, ctx->d_patch_fx))) return rc;
-            ctx->d_patch_fx = nullptr;
+            for (void *pp : {(void *)ctx->d_patch_fm, (void *)ctx->d_patch_fe, (void *)ctx->d_patch_edges,
+                             (void *)ctx->d_patch_eid})
+                if ((rc = dfree(ctx, pp))) return rc;
+            ctx->d_patch_fm = ctx->d_patch_fe = nullptr;
+            ctx->d_patch_edges = ctx->d_patch_eid = nullptr;
+            ctx->n_patch_edge = 0;
             for (auto &sh : ctx->shadow) {
                 if ((rc = dfree(ctx, sh))) return rc;
                 sh = nullptr;
@@ -582,11 +589,24 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
                                   ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));   // pidx ends with this block
             if ((rc = upload_replace(ctx, &ctx->d_patch, patch.data(), patch.size()))) return rc;
-            if (patch.size() < (size_t)kPatchSplitBelow) {       // face fluxes of the two-launch update
-                if ((rc = dalloc_replace(ctx, &ctx->d_patch_fx, (size_t)12 * patch.size()))) return rc;
-            } else {
-                if ((rc = dfree(ctx, ctx->d_patch_fx))) return rc;
-                ctx->d_patch_fx = nullptr;
+            {   // face-flux buffers of the two-launch update: the edge items are the (axis, cell)
+                // pairs whose +1 neighbour is no patch cell (their face +1/2 has no owner)
+                const int np = (int)patch.size();
+                std::vector<int> edges, eid((size_t)3 * np, -1);
+                for (int a = 0; a < 3; ++a) {
+                    const long long st = a == 0 ? 1LL : (a == 1 ? (long long)Nx : nxy);
+                    for (int q = 0; q < np; ++q)
+                        if (pidx[(size_t)(patch[(size_t)q] + st)] < 0) {
+                            eid[(size_t)a * np + q] = (int)edges.size();
+                            edges.push_back(a * np + q);
+                        }
+                }
+                ctx->n_patch_edge = (int)edges.size();
+                if ((rc = dalloc_replace(ctx, &ctx->d_patch_fm, (size_t)6 * np))) return rc;
+                if ((rc = dalloc_replace(ctx, &ctx->d_patch_fe, std::max<size_t>(1, 2 * edges.size())))) return rc;
+                if ((rc = upload_replace(ctx, &ctx->d_patch_edges, edges.data(), edges.size()))) return rc;
+                if ((rc = upload_replace(ctx, &ctx->d_patch_eid, eid.data(), eid.size()))) return rc;
+                HIPCHK(hipStreamSynchronize(ctx->stream));   // edges, eid are locals
             }
             for (int q = 0; q < 3; ++q) {
                 if ((rc = dalloc_replace(ctx, &ctx->shadow[q], (size_t)4 * patch.size()))) return rc;

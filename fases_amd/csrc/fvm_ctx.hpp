@@ -48,7 +48,12 @@ struct fases_ctx {
     int *d_patch = nullptr;              // their grid indices
     int *d_pidx = nullptr;               // grid -> patch index, -1 elsewhere (allocated on demand)
     double *shadow[3] = {};              // FP64 shadow per field set, [4][n_patch]
-    double *d_patch_fx = nullptr;        // face fluxes of the patch cells, [3 axes][n_patch][4]
+    // face fluxes of the two-launch patch update: face -1/2 of every (axis, cell) [3][n_patch][2],
+    // face +1/2 of the edge items [n_patch_edge][2]; edge items (a n_patch + q) and the item
+    // of each (axis, cell) [3][n_patch] (-1: its +1 neighbour is a patch cell)
+    double *d_patch_fm = nullptr, *d_patch_fe = nullptr;
+    int *d_patch_edges = nullptr, *d_patch_eid = nullptr;
+    int n_patch_edge = 0;
     std::vector<int> h_patch;
     // boundary list
     int nB = 0;
@@ -158,11 +163,37 @@ int dalloc_replace(fases_ctx *ctx, T **slot, size_t count) {
     return dalloc(ctx, slot, count);
 }
 
+// Frees every allocation of the context and clears every pointer that named one (a later
+// *_replace call must not meet a freed pointer).
 void dfree_all(fases_ctx *ctx, void *keep_none = nullptr) {
     (void)keep_none;
     for (auto &b : ctx->bufs) if (b.ptr) (void)hipFree(b.ptr);
     ctx->bufs.clear();
     ctx->dev_bytes = 0;
+    for (auto &set : ctx->U)
+        for (auto &f : set) f = nullptr;
+    ctx->d_upd = nullptr;
+    ctx->d_patch = ctx->d_pidx = nullptr;
+    for (auto &sh : ctx->shadow) sh = nullptr;
+    ctx->d_patch_fm = ctx->d_patch_fe = nullptr;
+    ctx->d_patch_edges = ctx->d_patch_eid = nullptr;
+    ctx->d_bcell = ctx->d_bmat = nullptr;
+    ctx->d_bdist = nullptr;
+    ctx->d_bnrm = nullptr;
+    ctx->d_Rinf = ctx->d_A = ctx->d_zeta = ctx->d_B = ctx->d_C = ctx->d_alpha = ctx->d_beta = nullptr;
+    ctx->d_Keff = ctx->d_Leff = nullptr;
+    for (int q = 0; q < 3; ++q) ctx->phi[q] = ctx->psi1[q] = ctx->psi2[q] = nullptr;
+    ctx->d_ade_rk = ctx->d_ade_aux = nullptr;
+    ctx->pml = PmlSet{};
+    ctx->d_pml = nullptr;
+    ctx->d_pml_cells = nullptr;
+    ctx->d_scell = nullptr;
+    ctx->d_sval = ctx->d_sfreq = ctx->d_us = nullptr;
+    ctx->d_rcell = nullptr;
+    ctx->d_ir = ctx->d_vel = nullptr;
+    ctx->d_slices = nullptr;
+    ctx->n_slices = 0;
+    ctx->d_amax = nullptr;
 }
 
 template <typename T>
@@ -448,11 +479,6 @@ bool rescale_due(long long n, int period) {
     return n % period == 0;
 }
 
-// Patches of fewer cells take the two-launch patch update (3 threads per cell): measured on
-// one source's 25^3 patch 24.2 -> 12.4 + 5.5 us per stage; on a 65^3 patch the split made
-// the 192x160x128 step slower (0.424 -> 0.448 ms: the one-thread grid already fills the
-// chip, and the split adds the face-flux round trip).
-constexpr int kPatchSplitBelow = 65536;
 
 // The boundary list of a context as the boundary kernels take it.
 inline Bnd bnd_of(const fases_ctx *ctx) {
@@ -523,36 +549,23 @@ int launch_stages(fases_ctx *ctx, long long n, const FieldSet<S> sets[4], double
             if (rc) return rc;
         }
         if (kev) HIPCHK(hipEventRecord(e1, ctx->stream));
-        if (ctx->n_patch >= kPatchSplitBelow) {
-            // FP64 update of a large patch: one thread per cell
-            const dim3 pg((ctx->n_patch + 63) / 64), pb(64);
+        if (ctx->n_patch > 0) {
+            // FP64 update of the patch: every face flux once, then the stage update per cell
+            const long long nf = 3LL * ctx->n_patch + ctx->n_patch_edge;
+            const dim3 fg((unsigned)((nf + 63) / 64)), pg((ctx->n_patch + 63) / 64), pb(64);
             const PatchIO<S> pio{sets[q.in], sets[0], sets[q.out], ctx->d_pidx, sh[q.in], sh[0], sh[q.out],
                                  ctx->n_patch, scd};
-            if (s == 0)
-                hipLaunchKernelGGL((k_flux_patch<S, 0>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.rho, P.c, P.h, P.dt);
-            else if (s == 1)
-                hipLaunchKernelGGL((k_flux_patch<S, 1>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.rho, P.c, P.h, P.dt);
-            else
-                hipLaunchKernelGGL((k_flux_patch<S, 2>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.rho, P.c, P.h, P.dt);
-        } else if (ctx->n_patch > 0) {
-            // FP64 update of a small patch: face fluxes per (cell, axis), then the stage update
-            const dim3 ag((unsigned)((3LL * ctx->n_patch + 63) / 64)), pg((ctx->n_patch + 63) / 64), pb(64);
-            const PatchIO<S> pio{sets[q.in], sets[0], sets[q.out], ctx->d_pidx, sh[q.in], sh[0], sh[q.out],
-                                 ctx->n_patch, scd};
-            hipLaunchKernelGGL((k_flux_patch_axis<S>), ag, pb, 0, ctx->stream, g, pio, ctx->d_patch, P.rho, P.c,
-                               ctx->d_patch_fx);
+            hipLaunchKernelGGL((k_flux_patch_faces<S>), fg, pb, 0, ctx->stream, g, pio, ctx->d_patch,
+                               ctx->d_patch_edges, ctx->n_patch_edge, P.rho, P.c, ctx->d_patch_fm, ctx->d_patch_fe);
             if (s == 0)
                 hipLaunchKernelGGL((k_flux_patch_update<S, 0>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.h, P.dt, ctx->d_patch_fx);
+                                   P.h, P.dt, ctx->d_patch_fm, ctx->d_patch_fe, ctx->d_patch_eid);
             else if (s == 1)
                 hipLaunchKernelGGL((k_flux_patch_update<S, 1>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.h, P.dt, ctx->d_patch_fx);
+                                   P.h, P.dt, ctx->d_patch_fm, ctx->d_patch_fe, ctx->d_patch_eid);
             else
                 hipLaunchKernelGGL((k_flux_patch_update<S, 2>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.h, P.dt, ctx->d_patch_fx);
+                                   P.h, P.dt, ctx->d_patch_fm, ctx->d_patch_fe, ctx->d_patch_eid);
         }
         if (ctx->n_src > 0)
             hipLaunchKernelGGL((k_sources<S, R>), dim3((ctx->n_src + sb - 1) / sb), dim3(sb), 0, ctx->stream,

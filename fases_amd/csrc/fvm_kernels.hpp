@@ -71,24 +71,32 @@ template <typename S> struct PatchIO {
 
 // Fluxes through the faces C+1/2 (P) and C-1/2 (M) of cell (i, j, k) along axis a (stride
 // st): pressure (Fp_p, Fp_m) and the axis velocity fu (Fu_p, Fu_m).
+// Godunov flux of p and of the axis velocity fu through one face of cell C along stride
+// st: the face -1/2 (side 0, window C-3st .. C+2st) or +1/2 (side 1, C-2st .. C+3st), the
+// reference's per-face reconstruction (functions3D.cuh:763-1001).
+template <typename IO, typename R>
+__device__ inline void face_flux(long long C, const IO &io, int fu, long long st, int side, R rho, R c, R &Fp, R &Fu) {
+#pragma clang fp contract(off)
+    R pw[6], uw[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        pw[q] = io.ld_in(0, C + (q - 3 + side) * st);
+        uw[q] = io.ld_in(fu, C + (q - 3 + side) * st);
+    }
+    R pl, pr, ul, ur;
+    weno_face(pw[0], pw[1], pw[2], pw[3], pw[4], pw[5], pl, pr);
+    weno_face(uw[0], uw[1], uw[2], uw[3], uw[4], uw[5], ul, ur);
+    godunov(pl, pr, ul, ur, rho, c, Fp, Fu);
+}
+
+// The four face fluxes of cell (i, j, k) along one axis: p and the axis velocity fu
+// (1 = u, 2 = v, 3 = w) through the faces +1/2 and -1/2.
 template <typename IO, typename R>
 __device__ inline void flux_axis_cell(const Grid &g, int i, int j, int k, const IO &io, int fu, long long st, R rho,
                                       R c, R &Fp_p, R &Fp_m, R &Fu_p, R &Fu_m) {
-#pragma clang fp contract(off)
     const long long C = (long long)i + (long long)g.Nx * j + g.nxy * k;
-    R pw[7], uw[7];
-#pragma unroll
-    for (int q = 0; q < 7; ++q) {
-        pw[q] = io.ld_in(0, C + (q - 3) * st);
-        uw[q] = io.ld_in(fu, C + (q - 3) * st);
-    }
-    R pl, pr, ul, ur;
-    weno_face(pw[1], pw[2], pw[3], pw[4], pw[5], pw[6], pl, pr);   // face +1/2
-    weno_face(uw[1], uw[2], uw[3], uw[4], uw[5], uw[6], ul, ur);
-    godunov(pl, pr, ul, ur, rho, c, Fp_p, Fu_p);
-    weno_face(pw[0], pw[1], pw[2], pw[3], pw[4], pw[5], pl, pr);   // face -1/2
-    weno_face(uw[0], uw[1], uw[2], uw[3], uw[4], uw[5], ul, ur);
-    godunov(pl, pr, ul, ur, rho, c, Fp_m, Fu_m);
+    face_flux(C, io, fu, st, 1, rho, c, Fp_p, Fu_p);
+    face_flux(C, io, fu, st, 0, rho, c, Fp_m, Fu_m);
 }
 
 // Stage update of cell (i, j, k) from its twelve face fluxes (x: F, y: G, z: H).
@@ -171,34 +179,51 @@ k_flux_patch(Grid g, PatchIO<S> io, const int *__restrict__ cells, PmlSet pml, d
     const int i = C % g.Nx, j = (C / g.Nx) % g.Ny, k = (int)(C / g.nxy);
     flux_cell<PatchIO<S>, double, STAGE>(g, i, j, k, io, pml, rho, c, h, dt);
 }
-// ... or in two launches (small patches: one source's default patch is 25^3 cells, a grid
-// too small to hide the latency of one thread per cell): one thread per (cell, axis)
-// forms that axis' four face fluxes into fx[(a * np + t) * 4 ..], then one thread per cell
-// applies the stage update. The same functions in the same order as flux_cell, so the
-// same bits either way.
+// ... or in two launches with every face flux formed once: one thread per (cell, axis)
+// forms the flux through the cell's face -1/2 into fm[(a np + q) 2 ..], plus one thread per
+// listed edge item e (edges[e] = a np + q: cell q, axis a, whose +1 neighbour is no patch
+// cell) the flux through its face +1/2 into fe[2 e ..]; then one thread per cell applies
+// the stage update, taking the face +1/2 from its neighbour's fm (the same window, so the
+// same bits) or from fe (eid[a np + q] = e).
 template <typename S>
 __global__ void __launch_bounds__(64)
-k_flux_patch_axis(Grid g, PatchIO<S> io, const int *__restrict__ cells, double rho, double c, double *__restrict__ fx) {
+k_flux_patch_faces(Grid g, PatchIO<S> io, const int *__restrict__ cells, const int *__restrict__ edges, int n_edge,
+                   double rho, double c, double *__restrict__ fm, double *__restrict__ fe) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * io.np) return;
-    const int a = t / io.np, q = t - a * io.np;
-    const int C = cells[q];
-    const int i = C % g.Nx, j = (C / g.Nx) % g.Ny, k = (int)(C / g.nxy);
+    const int n3 = 3 * io.np;
+    if (t >= n3 + n_edge) return;
+    int a, q, side;
+    double *o;
+    const int u = t < n3 ? t : edges[t - n3];  // a np + q
+    a = u / io.np; q = u - a * io.np;
+    if (t < n3) {
+        side = 0; o = fm + 2LL * t;
+    } else {
+        side = 1; o = fe + 2LL * (t - n3);
+    }
     const long long st = a == 0 ? 1LL : (a == 1 ? (long long)g.Nx : g.nxy);
-    double *o = fx + 4LL * t;
-    flux_axis_cell(g, i, j, k, io, a + 1, st, rho, c, o[0], o[1], o[2], o[3]);
+    face_flux((long long)cells[q], io, a + 1, st, side, rho, c, o[0], o[1]);
 }
 template <typename S, int STAGE>
 __global__ void __launch_bounds__(64)
 k_flux_patch_update(Grid g, PatchIO<S> io, const int *__restrict__ cells, PmlSet pml, double h, double dt,
-                    const double *__restrict__ fx) {
+                    const double *__restrict__ fm, const double *__restrict__ fe, const int *__restrict__ eid) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= io.np) return;
     const int C = cells[t];
     const int i = C % g.Nx, j = (C / g.Nx) % g.Ny, k = (int)(C / g.nxy);
-    const double *x = fx + 4LL * t, *y = fx + 4LL * (io.np + t), *z = fx + 4LL * (2LL * io.np + t);
-    flux_cell_update<PatchIO<S>, double, STAGE>(g, i, j, k, io, pml, h, dt, x[0], x[1], x[2], x[3], y[0], y[1], y[2],
-                                                y[3], z[0], z[1], z[2], z[3]);
+    double F[3][4];                             // per axis: Fp +1/2, Fp -1/2, Fu +1/2, Fu -1/2
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const long long st = a == 0 ? 1LL : (a == 1 ? (long long)g.Nx : g.nxy);
+        const double *m = fm + 2LL * ((long long)a * io.np + t);
+        const int nb = io.pidx[C + st];
+        const double *pp = nb >= 0 ? fm + 2LL * ((long long)a * io.np + nb) : fe + 2LL * eid[(long long)a * io.np + t];
+        F[a][0] = pp[0]; F[a][1] = m[0]; F[a][2] = pp[1]; F[a][3] = m[1];
+    }
+    flux_cell_update<PatchIO<S>, double, STAGE>(g, i, j, k, io, pml, h, dt, F[0][0], F[0][1], F[0][2], F[0][3],
+                                                F[1][0], F[1][1], F[1][2], F[1][3], F[2][0], F[2][1], F[2][2],
+                                                F[2][3]);
 }
 
 // Copies the stored fields of the patch cells into an FP64 shadow set.

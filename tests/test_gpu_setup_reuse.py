@@ -105,3 +105,29 @@ def test_duplicate_source_cells_get_the_merged_gradient():
     np.testing.assert_allclose(ir_d, ir_1, rtol=1e-14, atol=1e-300)
     np.testing.assert_allclose(gs_d, [gs_1[0], gs_1[0]], rtol=1e-12)
     np.testing.assert_allclose(gm_d, gm_1, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_second_geometry_on_one_context_equals_fresh_context(prec):
+    """set_geometry on a context that has run (sources, FP64 source region, receivers, FP16
+    scales all set up) starts it afresh: the run after it equals a fresh context's."""
+    announce("set_geometry twice on one context, %s" % prec)
+    wl = room(60)
+    sc = wl.scene
+
+    def load(sv):
+        sv.set_geometry(sc.S, sc.oid, sc.dims)
+        sv.set_materials(wl.materials)
+        sv.set_sources([wl.cell(s) for s in wl.src_ijk], wl.src_sval, wl.src_opt, wl.u_s)
+        sv.set_receivers([wl.cell(r) for r in wl.rec_ijk])
+        sv.set_fields(wl.initial_pressure())
+
+    sv = W.make_solver(wl, prec)
+    try:
+        ir0 = sv.run(wl.nsteps)[0]
+        load(sv)
+        ir1 = sv.run(wl.nsteps)[0]
+    finally:
+        sv.close()
+    assert np.abs(ir0).max() > 0
+    np.testing.assert_array_equal(ir1, ir0)
