@@ -336,8 +336,6 @@ int fases_set_geometry(fases_ctx *ctx, const float *S, const uint8_t *oid, int N
     ctx->n_patch = 0;
     ctx->d_patch = nullptr;
     ctx->d_pidx = nullptr;
-    ctx->d_patch_fm = ctx->d_patch_fe = nullptr;      // (freed with every other buffer above)
-    ctx->d_patch_edges = ctx->d_patch_eid = nullptr;
     ctx->n_patch_edge = 0;
     ctx->h_patch.clear();
     for (auto &sh : ctx->shadow) sh = nullptr;
@@ -572,10 +570,10 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
             if ((rc = dfree(ctx, ctx->d_patch))) return rc;
             ctx->d_patch = nullptr;
             for (void *pp : {(void *)ctx->d_patch_fm, (void *)ctx->d_patch_fe, (void *)ctx->d_patch_edges,
-                             (void *)ctx->d_patch_eid})
+                             (void *)ctx->d_patch_eid, (void *)ctx->d_patch_src})
                 if ((rc = dfree(ctx, pp))) return rc;
             ctx->d_patch_fm = ctx->d_patch_fe = nullptr;
-            ctx->d_patch_edges = ctx->d_patch_eid = nullptr;
+            ctx->d_patch_edges = ctx->d_patch_eid = ctx->d_patch_src = nullptr;
             ctx->n_patch_edge = 0;
             for (auto &sh : ctx->shadow) {
                 if ((rc = dfree(ctx, sh))) return rc;
@@ -606,7 +604,21 @@ int fases_set_sources(fases_ctx *ctx, int n, const int64_t *cell, const double *
                 if ((rc = dalloc_replace(ctx, &ctx->d_patch_fe, std::max<size_t>(1, 2 * edges.size())))) return rc;
                 if ((rc = upload_replace(ctx, &ctx->d_patch_edges, edges.data(), edges.size()))) return rc;
                 if ((rc = upload_replace(ctx, &ctx->d_patch_eid, eid.data(), eid.size()))) return rc;
-                HIPCHK(hipStreamSynchronize(ctx->stream));   // edges, eid are locals
+                // every source in the patch: the patch update injects them (no k_sources launch)
+                std::vector<int> psrc((size_t)np, -1);
+                bool all_in = !mc.empty();
+                for (size_t q = 0; q < mc.size(); ++q) {
+                    const int w = pidx[(size_t)mc[q]];
+                    if (w < 0) all_in = false;
+                    else psrc[(size_t)w] = (int)q;
+                }
+                if (all_in) {
+                    if ((rc = upload_replace(ctx, &ctx->d_patch_src, psrc.data(), psrc.size()))) return rc;
+                } else {
+                    if ((rc = dfree(ctx, ctx->d_patch_src))) return rc;
+                    ctx->d_patch_src = nullptr;
+                }
+                HIPCHK(hipStreamSynchronize(ctx->stream));   // edges, eid, psrc are locals
             }
             for (int q = 0; q < 3; ++q) {
                 if ((rc = dalloc_replace(ctx, &ctx->shadow[q], (size_t)4 * patch.size()))) return rc;

@@ -54,6 +54,8 @@ struct fases_ctx {
     double *d_patch_fm = nullptr, *d_patch_fe = nullptr;
     int *d_patch_edges = nullptr, *d_patch_eid = nullptr;
     int n_patch_edge = 0;
+    int *d_patch_src = nullptr;          // source index of each patch cell (-1: none), when every
+                                         // source lies in the patch (its update injects them)
     std::vector<int> h_patch;
     // boundary list
     int nB = 0;
@@ -177,6 +179,7 @@ void dfree_all(fases_ctx *ctx, void *keep_none = nullptr) {
     for (auto &sh : ctx->shadow) sh = nullptr;
     ctx->d_patch_fm = ctx->d_patch_fe = nullptr;
     ctx->d_patch_edges = ctx->d_patch_eid = nullptr;
+    ctx->d_patch_src = nullptr;
     ctx->d_bcell = ctx->d_bmat = nullptr;
     ctx->d_bdist = nullptr;
     ctx->d_bnrm = nullptr;
@@ -557,17 +560,19 @@ int launch_stages(fases_ctx *ctx, long long n, const FieldSet<S> sets[4], double
                                  ctx->n_patch, scd};
             hipLaunchKernelGGL((k_flux_patch_faces<S>), fg, pb, 0, ctx->stream, g, pio, ctx->d_patch,
                                ctx->d_patch_edges, ctx->n_patch_edge, P.rho, P.c, ctx->d_patch_fm, ctx->d_patch_fe);
+            const PatchSrc ps{ctx->n_src > 0 ? ctx->d_patch_src : nullptr, ctx->d_sval, ctx->d_us,
+                              ctx->src_opt == FASES_SRC_SINE ? ctx->n_samp : 0LL, q.nn, q.w, P.dt, P.c, P.rho, As, P.h};
             if (s == 0)
                 hipLaunchKernelGGL((k_flux_patch_update<S, 0>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.h, P.dt, ctx->d_patch_fm, ctx->d_patch_fe, ctx->d_patch_eid);
+                                   P.h, P.dt, ctx->d_patch_fm, ctx->d_patch_fe, ctx->d_patch_eid, ps);
             else if (s == 1)
                 hipLaunchKernelGGL((k_flux_patch_update<S, 1>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.h, P.dt, ctx->d_patch_fm, ctx->d_patch_fe, ctx->d_patch_eid);
+                                   P.h, P.dt, ctx->d_patch_fm, ctx->d_patch_fe, ctx->d_patch_eid, ps);
             else
                 hipLaunchKernelGGL((k_flux_patch_update<S, 2>), pg, pb, 0, ctx->stream, g, pio, ctx->d_patch, ctx->pml,
-                                   P.h, P.dt, ctx->d_patch_fm, ctx->d_patch_fe, ctx->d_patch_eid);
+                                   P.h, P.dt, ctx->d_patch_fm, ctx->d_patch_fe, ctx->d_patch_eid, ps);
         }
-        if (ctx->n_src > 0)
+        if (ctx->n_src > 0 && !(ctx->n_patch > 0 && ctx->d_patch_src))
             hipLaunchKernelGGL((k_sources<S, R>), dim3((ctx->n_src + sb - 1) / sb), dim3(sb), 0, ctx->stream,
                                sets[q.out].p, ctx->d_scell, ctx->d_sval, ctx->d_sfreq, ctx->d_us,
                                ctx->src_opt == FASES_SRC_SINE ? ctx->n_samp : 0LL, ctx->n_src, ctx->src_opt, q.nn, q.w, P.dt, P.c, P.rho, As, P.h,

@@ -55,22 +55,25 @@ template <typename S> struct PatchIO {
     double *sh_out;
     int np;
     Scl<double> sc;
+    // entry f of a scale array by selection (an index that varies per lane would put the
+    // array in scratch memory)
+    __device__ static double pick(const double (&a)[4], int f) {
+        return f == 0 ? a[0] : (f == 1 ? a[1] : (f == 2 ? a[2] : a[3]));
+    }
     __device__ double ld_in(int f, long long C) const {
         const int q = pidx[C];
-        return q >= 0 ? sh_in[(long long)f * np + q] : Conv<S, double>::ld(fld(in, f)[C], sc.s[f]);
+        return q >= 0 ? sh_in[(long long)f * np + q] : Conv<S, double>::ld(fld(in, f)[C], pick(sc.s, f));
     }
     __device__ double ld_base(int f, long long C) const {
         const int q = pidx[C];
-        return q >= 0 ? sh_base[(long long)f * np + q] : Conv<S, double>::ld(fld(base, f)[C], sc.s[f]);
+        return q >= 0 ? sh_base[(long long)f * np + q] : Conv<S, double>::ld(fld(base, f)[C], pick(sc.s, f));
     }
     __device__ void st(int f, long long C, double v) const {
         sh_out[(long long)f * np + pidx[C]] = v;
-        fld(out, f)[C] = Conv<S, double>::st(v, sc.is[f]);
+        fld(out, f)[C] = Conv<S, double>::st(v, pick(sc.is, f));
     }
 };
 
-// Fluxes through the faces C+1/2 (P) and C-1/2 (M) of cell (i, j, k) along axis a (stride
-// st): pressure (Fp_p, Fp_m) and the axis velocity fu (Fu_p, Fu_m).
 // Godunov flux of p and of the axis velocity fu through one face of cell C along stride
 // st: the face -1/2 (side 0, window C-3st .. C+2st) or +1/2 (side 1, C-2st .. C+3st), the
 // reference's per-face reconstruction (functions3D.cuh:763-1001).
@@ -103,7 +106,7 @@ __device__ inline void flux_axis_cell(const Grid &g, int i, int j, int k, const 
 template <typename IO, typename R, int STAGE>
 __device__ inline void flux_cell_update(const Grid &g, int i, int j, int k, const IO &io, const PmlSet &pml, R h,
                                         R dt, R FpP, R FpM, R FuP, R FuM, R GpP, R GpM, R GvP, R GvM, R HpP, R HpM,
-                                        R HwP, R HwM) {
+                                        R HwP, R HwM, bool src = false, R src_add = R(0)) {
 #pragma clang fp contract(off)
     const long long C = (long long)i + (long long)g.Nx * j + g.nxy * k;
     const R p_in = io.ld_in(0, C), u_in = io.ld_in(1, C), v_in = io.ld_in(2, C), w_in = io.ld_in(3, C);
@@ -134,6 +137,7 @@ __device__ inline void flux_cell_update(const Grid &g, int i, int j, int k, cons
     if (pml.on)
         pml_apply<R, STAGE == 2>(pml, i, j, k, g.Nx, g.Ny, po, uo, vo, wo, (-FpP + FpM) / h, (-GpP + GpM) / h,
                                  (-HpP + HpM) / h, (-FuP + FuM) / h, (-GvP + GvM) / h, (-HwP + HwM) / h, wgt * dt);
+    if (src) po = po + src_add;         // a point source's injection (k_sources' arithmetic)
     io.st(1, C, uo);
     io.st(2, C, vo);
     io.st(3, C, wo);
@@ -204,12 +208,31 @@ k_flux_patch_faces(Grid g, PatchIO<S> io, const int *__restrict__ cells, const i
     const long long st = a == 0 ? 1LL : (a == 1 ? (long long)g.Nx : g.nxy);
     face_flux((long long)cells[q], io, a + 1, st, side, rho, c, o[0], o[1]);
 }
+// Point sources inside the patch, injected by the patch update instead of k_sources (the
+// same arithmetic, applied to the same FP64 value, so the same bits): source index of each
+// patch cell (psrc, -1: none; null: k_sources injects every source) and k_sources'
+// parameters of the stage.
+struct PatchSrc {
+    const int *psrc;
+    const double *sval, *u_s;
+    long long us_stride, nn;
+    double w, dt, c, rho, As, h;
+};
+
 template <typename S, int STAGE>
 __global__ void __launch_bounds__(64)
 k_flux_patch_update(Grid g, PatchIO<S> io, const int *__restrict__ cells, PmlSet pml, double h, double dt,
-                    const double *__restrict__ fm, const double *__restrict__ fe, const int *__restrict__ eid) {
+                    const double *__restrict__ fm, const double *__restrict__ fe, const int *__restrict__ eid,
+                    PatchSrc ps) {
+#pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= io.np) return;
+    const int sidx = ps.psrc ? ps.psrc[t] : -1;
+    double add = 0.0;
+    if (sidx >= 0) {
+        const double amp = ps.u_s[(long long)sidx * ps.us_stride + ps.nn];
+        add = ps.w * ps.dt * ps.c * ps.c * ps.sval[sidx] * amp * ps.rho * ps.As / (ps.h * ps.h * ps.h);
+    }
     const int C = cells[t];
     const int i = C % g.Nx, j = (C / g.Nx) % g.Ny, k = (int)(C / g.nxy);
     double F[3][4];                             // per axis: Fp +1/2, Fp -1/2, Fu +1/2, Fu -1/2
@@ -223,7 +246,7 @@ k_flux_patch_update(Grid g, PatchIO<S> io, const int *__restrict__ cells, PmlSet
     }
     flux_cell_update<PatchIO<S>, double, STAGE>(g, i, j, k, io, pml, h, dt, F[0][0], F[0][1], F[0][2], F[0][3],
                                                 F[1][0], F[1][1], F[1][2], F[1][3], F[2][0], F[2][1], F[2][2],
-                                                F[2][3]);
+                                                F[2][3], sidx >= 0, add);
 }
 
 // Copies the stored fields of the patch cells into an FP64 shadow set.
