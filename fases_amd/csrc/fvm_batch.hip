@@ -5,7 +5,7 @@
 // captured into a hipGraph (two steps, one per rotation parity) and replayed.
 //
 // The per-job arithmetic is exactly the single-context path's (same device functions:
-// flux_tile, flux_cell, boundary_cell), so a job gives the same IR batched or alone.
+// flux_tile, face_flux_pf, boundary_cell), so a job gives the same IR batched or alone.
 // Jobs may differ in grid, materials, sources, receivers and step count; jobs that
 // finish drop out at phase boundaries (the step counts sorted ascending).
 
@@ -102,7 +102,7 @@ k_patch_batch(const PatchJob<S> *__restrict__ jobs, const int *__restrict__ off,
     const PatchIO<S> io{fs(pi), fs(pb), fs(po), J.pidx, J.sh[pi], J.sh[pb], J.sh[po], J.np, J.sc};
     const int C = J.cells[lt];
     const int i = C % J.g.Nx, jj = (C / J.g.Nx) % J.g.Ny, k = (int)(C / J.g.nxy);
-    flux_cell<PatchIO<S>, double, STAGE>(J.g, i, jj, k, io, J.pml, J.rho, J.c, J.h, J.dt);
+    patch_cell<S, STAGE>(J.g, i, jj, k, io, J.pml, J.rho, J.c, J.h, J.dt);
 }
 
 template <typename S> struct SrcJob {

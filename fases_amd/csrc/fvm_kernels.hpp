@@ -102,6 +102,27 @@ __device__ inline void flux_axis_cell(const Grid &g, int i, int j, int k, const 
     face_flux(C, io, fu, st, 0, rho, c, Fp_m, Fu_m);
 }
 
+// The same face flux with the tiled kernel's reconstruction (product-form WENO5-Z weights,
+// one reciprocal per state) in FP64: the left state is cell C-1's (face -1/2) or C's (face
+// +1/2) A, the right state the next cell's B. The FP64 source patch of reduced-precision
+// storage uses this form; it agrees with the reference order to rounding, and a face's
+// flux depends only on its six cells, whichever cell's thread forms it.
+template <typename IO>
+__device__ inline void face_flux_pf(long long C, const IO &io, int fu, long long st, int side, double rho, double c,
+                                    double &Fp, double &Fu) {
+    double pw[6], uw[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        pw[q] = io.ld_in(0, C + (q - 3 + side) * st);
+        uw[q] = io.ld_in(fu, C + (q - 3 + side) * st);
+    }
+    const double pl = weno_left_of(pw[0], pw[1], pw[2], pw[3], pw[4]);
+    const double pr = weno_right_of(pw[1], pw[2], pw[3], pw[4], pw[5]);
+    const double ul = weno_left_of(uw[0], uw[1], uw[2], uw[3], uw[4]);
+    const double ur = weno_right_of(uw[1], uw[2], uw[3], uw[4], uw[5]);
+    godunov(pl, pr, ul, ur, rho, c, Fp, Fu);
+}
+
 // Stage update of cell (i, j, k) from its twelve face fluxes (x: F, y: G, z: H).
 template <typename IO, typename R, int STAGE>
 __device__ inline void flux_cell_update(const Grid &g, int i, int j, int k, const IO &io, const PmlSet &pml, R h,
@@ -171,17 +192,23 @@ k_flux_simple(Grid g, GridIO<S, R> io, const uint8_t *__restrict__ upd, PmlSet p
     flux_cell<GridIO<S, R>, R, STAGE>(g, i, j, k, io, pml, rho, c, h, dt);
 }
 
-// The same stage update in FP64 arithmetic and FP64 shadow storage on the patch cells:
-// one thread per cell (large patches) ...
+// The same stage update in FP64 arithmetic and FP64 shadow storage on the patch cells
+// (face_flux_pf's reconstruction): all six face fluxes of one cell (the batched loop, one
+// thread per cell) ...
 template <typename S, int STAGE>
-__global__ void __launch_bounds__(64)
-k_flux_patch(Grid g, PatchIO<S> io, const int *__restrict__ cells, PmlSet pml, double rho, double c, double h,
-             double dt) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= io.np) return;
-    const int C = cells[t];
-    const int i = C % g.Nx, j = (C / g.Nx) % g.Ny, k = (int)(C / g.nxy);
-    flux_cell<PatchIO<S>, double, STAGE>(g, i, j, k, io, pml, rho, c, h, dt);
+__device__ inline void patch_cell(const Grid &g, int i, int j, int k, const PatchIO<S> &io, const PmlSet &pml,
+                                  double rho, double c, double h, double dt) {
+    const long long C = (long long)i + (long long)g.Nx * j + g.nxy * k;
+    double F[3][4];                             // per axis: Fp +1/2, Fp -1/2, Fu +1/2, Fu -1/2
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const long long st = a == 0 ? 1LL : (a == 1 ? (long long)g.Nx : g.nxy);
+        face_flux_pf(C, io, a + 1, st, 1, rho, c, F[a][0], F[a][2]);
+        face_flux_pf(C, io, a + 1, st, 0, rho, c, F[a][1], F[a][3]);
+    }
+    flux_cell_update<PatchIO<S>, double, STAGE>(g, i, j, k, io, pml, h, dt, F[0][0], F[0][1], F[0][2], F[0][3],
+                                                F[1][0], F[1][1], F[1][2], F[1][3], F[2][0], F[2][1], F[2][2],
+                                                F[2][3]);
 }
 // ... or in two launches with every face flux formed once: one thread per (cell, axis)
 // forms the flux through the cell's face -1/2 into fm[(a np + q) 2 ..], plus one thread per
@@ -206,7 +233,7 @@ k_flux_patch_faces(Grid g, PatchIO<S> io, const int *__restrict__ cells, const i
         side = 1; o = fe + 2LL * (t - n3);
     }
     const long long st = a == 0 ? 1LL : (a == 1 ? (long long)g.Nx : g.nxy);
-    face_flux((long long)cells[q], io, a + 1, st, side, rho, c, o[0], o[1]);
+    face_flux_pf((long long)cells[q], io, a + 1, st, side, rho, c, o[0], o[1]);
 }
 // Point sources inside the patch, injected by the patch update instead of k_sources (the
 // same arithmetic, applied to the same FP64 value, so the same bits): source index of each
