@@ -451,8 +451,10 @@ int rescale_fields(fases_ctx *ctx) {
         if (rc) return rc;
     }
     hipLaunchKernelGGL((k_fill<unsigned>), dim3(1), dim3(64), 0, ctx->stream, ctx->d_amax, 5, 0u);
+    // 1024 blocks: 16 waves per CU stream the set, and 5 atomics per block meet in L2
+    const int nb_max = (int)std::min<long long>(1024, (ctx->N + 255) / 256);
+    hipLaunchKernelGGL((k_absmax<S>), dim3(nb_max), dim3(256), 0, ctx->stream, fset<S>(ctx, 0), ctx->N, ctx->d_amax);
     const int nb = (int)std::min<long long>(4096, (ctx->N + 255) / 256);
-    hipLaunchKernelGGL((k_absmax<S>), dim3(nb), dim3(256), 0, ctx->stream, fset<S>(ctx, 0), ctx->N, ctx->d_amax);
     unsigned hm[5];
     HIPCHK(hipMemcpyAsync(hm, ctx->d_amax, sizeof hm, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));

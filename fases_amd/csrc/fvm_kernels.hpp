@@ -289,25 +289,68 @@ __global__ void k_patch_gather(FieldSet<S> f, const int *__restrict__ cells, int
 // Largest |stored value| of each field of one set and a non-finite flag:
 // out[0..3] = float bit patterns (non-negative floats order like unsigned ints), out[4] != 0
 // when a value is Inf or NaN. out[] must be zero on entry.
+// 16-byte groups of stored values: the reduction and the rescale move whole groups per
+// load and store when all four fields start on a 16-byte boundary (hipMalloc'd fields
+// do); the cells past the last whole group, or every cell otherwise, go one at a time.
+template <typename S> struct alignas(16) Pack16 {
+    S v[16 / sizeof(S)];
+};
+template <typename S> __device__ inline long long pack16_groups(const FieldSet<S> &f, long long n) {
+    const unsigned long long a = (unsigned long long)f.p | (unsigned long long)f.u | (unsigned long long)f.v |
+                                 (unsigned long long)f.w;
+    return (a & 15ULL) ? 0LL : n / (long long)(16 / sizeof(S));
+}
+
 template <typename S>
 __global__ void __launch_bounds__(256) k_absmax(FieldSet<S> f, long long n, unsigned *__restrict__ out) {
+    constexpr int V = 16 / (int)sizeof(S);
     float m[4] = {0.f, 0.f, 0.f, 0.f};
     unsigned bad = 0;
-    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    auto take = [&](int q, S v) {
+        const float x = fabsf((float)v);
+        if (!(x <= 3.0e38f)) bad = 1;
+        else m[q] = fmaxf(m[q], x);
+    };
+    const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x, stride = (long long)gridDim.x * blockDim.x;
+    const long long ng = pack16_groups(f, n);
+    for (long long c = t0; c < ng; c += stride) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float x = fabsf((float)fld(f, q)[c]);
-            if (!(x <= 3.0e38f)) bad = 1;
-            else m[q] = fmaxf(m[q], x);
+            const Pack16<S> w = reinterpret_cast<const Pack16<S> *>(fld(f, q))[c];
+#pragma unroll
+            for (int e = 0; e < V; ++e) take(q, w.v[e]);
         }
+    }
+    for (long long c = ng * V + t0; c < n; c += stride) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) take(q, fld(f, q)[c]);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q)
         for (int o = 32; o > 0; o >>= 1) m[q] = fmaxf(m[q], __shfl_xor(m[q], o, 64));
+    // the block's waves through LDS, then one atomic per field and block (atomics from
+    // every wave on the same five words serialise: 0.8 ms on a 53 M-cell set)
+    __shared__ float wm[4][4];
+    __shared__ unsigned wb[4];
+    const int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) atomicMax(out + q, __float_as_uint(m[q]));
-        if (bad) atomicOr(out + 4, 1u);
+        for (int q = 0; q < 4; ++q) wm[wv][q] = m[q];
+    }
+    unsigned any_bad = __any(bad) ? 1u : 0u;
+    if ((threadIdx.x & 63) == 0) wb[wv] = any_bad;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x + 63) >> 6;
+        unsigned b = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float x = wm[0][q];
+            for (int w = 1; w < nw; ++w) x = fmaxf(x, wm[w][q]);
+            atomicMax(out + q, __float_as_uint(x));
+        }
+        for (int w = 0; w < nw; ++w) b |= wb[w];
+        if (b) atomicOr(out + 4, 1u);
     }
 }
 
@@ -319,7 +362,20 @@ template <typename T> __global__ void k_fill(T *__restrict__ p, int n, T v) {
 // Multiplies the stored values of each field by a power of two (exact while in range).
 template <typename S>
 __global__ void __launch_bounds__(256) k_rescale(FieldSet<S> f, long long n, Scl<float> fac) {
-    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    constexpr int V = 16 / (int)sizeof(S);
+    const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x, stride = (long long)gridDim.x * blockDim.x;
+    const long long ng = pack16_groups(f, n);
+    for (long long c = t0; c < ng; c += stride) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            Pack16<S> *a = reinterpret_cast<Pack16<S> *>(fld(f, q));
+            Pack16<S> w = a[c];
+#pragma unroll
+            for (int e = 0; e < V; ++e) w.v[e] = (S)((float)w.v[e] * fac.s[q]);
+            a[c] = w;
+        }
+    }
+    for (long long c = ng * V + t0; c < n; c += stride) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             S *a = fld(f, q);
