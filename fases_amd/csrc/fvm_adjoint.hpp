@@ -351,7 +351,6 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
     const bool halo_wave = wy < 2 || wy > NW - 3;
     const R bdh = K.bdh;
     auto plane = [&](int k) -> long long { return nxy * (long long)max(0, min(k, Nz - 1)); };
-    auto lt = [&](const R *a, long long q) -> R { return upd[q] ? bdh * a[q] : R(0); };
 
     // halo items of this thread: items tid, tid + 64 NW, ... (see the tile shapes above):
     // vp rows 0, 1, NW+2, NW+3 and the four side columns of each end of the own rows,
@@ -450,10 +449,55 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
         }
         ap[3] = ap[4] = aw[3] = aw[4] = R(0);
     }
+    // Plane k's own-column values and halo items, loaded a step ahead (raw, with the update
+    // flag where the value is an output adjoint) so that their latency hides behind a whole
+    // step of work instead of stalling the LDS fill before the first barrier.
+    S pf_vu = S(), pf_vv = S();
+    R pf_lu = R(0), pf_lv = R(0);
+    unsigned pf_f = 0;
+    S pf_hs[NH];
+    R pf_hr[NH];
+    unsigned pf_hf[NH];
+#pragma unroll
+    for (int hq = 0; hq < NH; ++hq) { pf_hs[hq] = S(); pf_hr[hq] = R(0); pf_hf[hq] = 0; }
+    auto prefetch = [&](int kk) __device__ __attribute__((always_inline)) {
+        const long long pk = nxy * (long long)kk;
+        pf_vu = PlaneIO<S>::ld(plane_rsrc(V.u + pk, pS), cS);
+        pf_vv = PlaneIO<S>::ld(plane_rsrc(V.v + pk, pS), cS);
+        pf_f = uld(pk);
+        pf_lu = rld(L1, pk);
+        pf_lv = rld(L2, pk);
+#pragma unroll
+        for (int hq = 0; hq < NH; ++hq) {
+            const long long q = pk + h_off[hq];
+            switch (h_kind[hq]) {             // wave-uniform
+            case 0: pf_hs[hq] = V.p[q]; break;
+            case 1: pf_hs[hq] = V.u[q]; break;
+            case 2: pf_hs[hq] = V.v[q]; break;
+            case 3: pf_hr[hq] = L0[q]; pf_hf[hq] = upd[q]; break;
+            case 4: pf_hr[hq] = L1[q]; pf_hf[hq] = upd[q]; break;
+            case 5: pf_hr[hq] = L2[q]; pf_hf[hq] = upd[q]; break;
+            default: break;
+            }
+        }
+    };
+    if (k0 < kend) prefetch(k0);
     auto step = [&](int k, auto phase) __device__ __attribute__((always_inline)) {
         constexpr int PH = decltype(phase)::value;
         constexpr int Z0 = PH % 5, Z1 = (PH + 1) % 5, Z2 = (PH + 2) % 5, Z3 = (PH + 3) % 5, Z4 = (PH + 4) % 5;
         const bool xy = (k >= k0 && k < kend);                 // block-uniform
+        // the values the completion of plane k - 2 reads, loaded at the start of the step
+        const int m = k - 2;
+        const bool done = out_cell && m >= k0 && m < kend;
+        const long long pm = nxy * (long long)m;
+        bool me = false;
+        R o0 = 0, o1 = 0, o2 = 0, o3 = 0, i0 = 0, i1 = 0, i2 = 0, i3 = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+        if (done) {
+            me = uld(pm) != 0;
+            o0 = rld(L0, pm); o1 = rld(L1, pm); o2 = rld(L2, pm); o3 = rld(L3, pm);
+            if (ACC_IN) { i0 = rld(lin0, pm); i1 = rld(lin1, pm); i2 = rld(lin2, pm); i3 = rld(lin3, pm); }
+            if (HAS_BASE && ACC_BASE) { c0 = rld(lb0, pm); c1 = rld(lb1, pm); c2 = rld(lb2, pm); c3 = rld(lb3, pm); }
+        }
         // lead loads for the next plane's windows
         const R np_ = vld(V.p, k + 3, sc.s[0]);
         const R nw_ = vld(V.w, k + 3, sc.s[3]);
@@ -464,26 +508,23 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
             naw = ltp(L3, p2);
         }
         if (xy) {
-            const long long pk = nxy * (long long)k;
             T.vp[wy + 2][lx + 4] = zp[Z2];
-            T.vu[wy][lx + 4] = vld(V.u, k, sc.s[1]);
-            T.vv[wy + 2][lx] = vld(V.v, k, sc.s[2]);
+            T.vu[wy][lx + 4] = CV::ld(pf_vu, sc.s[1]);
+            T.vv[wy + 2][lx] = CV::ld(pf_vv, sc.s[2]);
             T.lp[wy + 1][lx + 3] = ap[Z1];
-            T.lu[wy][lx + 3] = ltp(L1, pk);
-            T.lv[wy + 1][lx] = ltp(L2, pk);
+            T.lu[wy][lx + 3] = pf_f ? bdh * pf_lu : R(0);
+            T.lv[wy + 1][lx] = pf_f ? bdh * pf_lv : R(0);
 #pragma unroll
             for (int hq = 0; hq < NH; ++hq) {
-                const long long q = pk + h_off[hq];
                 switch (h_kind[hq]) {             // wave-uniform
-                case 0: *h_dst[hq] = CV::ld(V.p[q], sc.s[0]); break;
-                case 1: *h_dst[hq] = CV::ld(V.u[q], sc.s[1]); break;
-                case 2: *h_dst[hq] = CV::ld(V.v[q], sc.s[2]); break;
-                case 3: *h_dst[hq] = lt(L0, q); break;
-                case 4: *h_dst[hq] = lt(L1, q); break;
-                case 5: *h_dst[hq] = lt(L2, q); break;
+                case 0: *h_dst[hq] = CV::ld(pf_hs[hq], sc.s[0]); break;
+                case 1: *h_dst[hq] = CV::ld(pf_hs[hq], sc.s[1]); break;
+                case 2: *h_dst[hq] = CV::ld(pf_hs[hq], sc.s[2]); break;
+                case 3: case 4: case 5: *h_dst[hq] = pf_hf[hq] ? bdh * pf_hr[hq] : R(0); break;
                 default: break;
                 }
             }
+            if (k + 1 < kend) prefetch(k + 1);
             __syncthreads();
             // ---- y pairs of every row
             {
@@ -589,26 +630,22 @@ k_flux_adj_tiled(int Nx, int Ny, int Nz, FieldSet<S> V, Scl<R> sc, const R *__re
             zb[Z0] += gw[0]; zb[Z1] += gw[1]; zb[Z2] += gw[2]; zb[Z3] += gw[3]; zb[Z4] += gw[4];
         }
         // ---- plane k - 2 is complete
-        const int m = k - 2;
-        if (out_cell && m >= k0 && m < kend) {
-            const long long pm = nxy * (long long)m;
-            const bool me = uld(pm) != 0;
-            const R o0 = rld(L0, pm), o1 = rld(L1, pm), o2 = rld(L2, pm), o3 = rld(L3, pm);
+        if (done) {
             const R v0 = xa[Z0][0] + za[Z0] + (me ? K.beta * o0 : R(0));
             const R v1 = xa[Z0][1] + (me ? K.beta * o1 : R(0));
             const R v2 = xa[Z0][2] + (me ? K.beta * o2 : R(0));
             const R v3 = zb[Z0] + (me ? K.beta * o3 : R(0));
-            rst(lin0, pm, ACC_IN ? rld(lin0, pm) + v0 : v0);
-            rst(lin1, pm, ACC_IN ? rld(lin1, pm) + v1 : v1);
-            rst(lin2, pm, ACC_IN ? rld(lin2, pm) + v2 : v2);
-            rst(lin3, pm, ACC_IN ? rld(lin3, pm) + v3 : v3);
+            rst(lin0, pm, ACC_IN ? i0 + v0 : v0);
+            rst(lin1, pm, ACC_IN ? i1 + v1 : v1);
+            rst(lin2, pm, ACC_IN ? i2 + v2 : v2);
+            rst(lin3, pm, ACC_IN ? i3 + v3 : v3);
             if (HAS_BASE) {
                 const R w0 = me ? K.alpha * o0 : R(0), w1 = me ? K.alpha * o1 : R(0);
                 const R w2 = me ? K.alpha * o2 : R(0), w3 = me ? K.alpha * o3 : R(0);
-                rst(lb0, pm, ACC_BASE ? rld(lb0, pm) + w0 : w0);
-                rst(lb1, pm, ACC_BASE ? rld(lb1, pm) + w1 : w1);
-                rst(lb2, pm, ACC_BASE ? rld(lb2, pm) + w2 : w2);
-                rst(lb3, pm, ACC_BASE ? rld(lb3, pm) + w3 : w3);
+                rst(lb0, pm, ACC_BASE ? c0 + w0 : w0);
+                rst(lb1, pm, ACC_BASE ? c1 + w1 : w1);
+                rst(lb2, pm, ACC_BASE ? c2 + w2 : w2);
+                rst(lb3, pm, ACC_BASE ? c3 + w3 : w3);
             }
         }
         // ---- advance the rings: logical slot 0 leaves, the new entries take free slots
