@@ -190,7 +190,8 @@ struct PatchAdjIn {
     Scl<double> sc;
 };
 
-// pv layout: [(ax * nD + d) * 10 + m], m < 5 the p-pair window slots, m >= 5 the velocity's.
+// pv layout: [(ax * 10 + m) * nD + d], m < 5 the p-pair window slots, m >= 5 the velocity's
+// (slot-major, so that the lanes of a wave, consecutive d, read and write consecutive words).
 template <typename S, typename R>
 __global__ void __launch_bounds__(64)
 k_flux_adj_patch_part(Grid g, PatchAdjIn<S> in, const R *__restrict__ L0, const R *__restrict__ L1,
@@ -229,11 +230,10 @@ k_flux_adj_patch_part(Grid g, PatchAdjIn<S> in, const R *__restrict__ L0, const 
         weno_pair_vjp(vv(fu, Cq - 2 * s), vv(fu, Cq - s), vv(fu, Cq), vv(fu, Cq + s), vv(fu, Cq + 2 * s), muL, muR,
                       gu);
     }
-    double *o = pv + ((long long)ax * nD + d) * 10;
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        o[m] = gp[m];
-        o[5 + m] = gu[m];
+        pv[((long long)ax * 10 + m) * nD + d] = gp[m];
+        pv[((long long)ax * 10 + 5 + m) * nD + d] = gu[m];
     }
 }
 
@@ -264,9 +264,9 @@ k_flux_adj_patch(Grid g, const R *__restrict__ L0, const R *__restrict__ L1, con
         const int cpos = pos[ax] - q;
         if (cpos >= 2 && cpos <= nax[ax] - 3) {
             const long long s = ax == 0 ? 1LL : (ax == 1 ? (long long)g.Nx : g.nxy);
-            const double *o = pv + ((long long)ax * nD + dmap[C - q * s]) * 10;
-            cp = o[q + 2];
-            cu = o[5 + q + 2];
+            const long long dd = dmap[C - q * s];
+            cp = pv[((long long)ax * 10 + q + 2) * nD + dd];
+            cu = pv[((long long)ax * 10 + 5 + q + 2) * nD + dd];
         }
         acc[0] += cp;
         acc[1 + ax] += cu;
