@@ -74,6 +74,20 @@ template <typename S> struct PatchIO {
     }
 };
 
+// The same access for the stage update of one patch cell, whose patch index q is known:
+// flux_cell_update touches only its own cell C, whose shadow entry is q (pidx[C] == q), so
+// the index lookups of PatchIO go away; the same values, the same bits.
+template <typename S> struct PatchOwnIO {
+    const PatchIO<S> &io;
+    int q;
+    __device__ double ld_in(int f, long long) const { return io.sh_in[(long long)f * io.np + q]; }
+    __device__ double ld_base(int f, long long) const { return io.sh_base[(long long)f * io.np + q]; }
+    __device__ void st(int f, long long C, double v) const {
+        io.sh_out[(long long)f * io.np + q] = v;
+        fld(io.out, f)[C] = Conv<S, double>::st(v, PatchIO<S>::pick(io.sc.is, f));
+    }
+};
+
 // Godunov flux of p and of the axis velocity fu through one face of cell C along stride
 // st: the face -1/2 (side 0, window C-3st .. C+2st) or +1/2 (side 1, C-2st .. C+3st), the
 // reference's per-face reconstruction (functions3D.cuh:763-1001).
@@ -271,9 +285,10 @@ k_flux_patch_update(Grid g, PatchIO<S> io, const int *__restrict__ cells, PmlSet
         const double *pp = nb >= 0 ? fm + 2LL * ((long long)a * io.np + nb) : fe + 2LL * eid[(long long)a * io.np + t];
         F[a][0] = pp[0]; F[a][1] = m[0]; F[a][2] = pp[1]; F[a][3] = m[1];
     }
-    flux_cell_update<PatchIO<S>, double, STAGE>(g, i, j, k, io, pml, h, dt, F[0][0], F[0][1], F[0][2], F[0][3],
-                                                F[1][0], F[1][1], F[1][2], F[1][3], F[2][0], F[2][1], F[2][2],
-                                                F[2][3], sidx >= 0, add);
+    const PatchOwnIO<S> own{io, t};
+    flux_cell_update<PatchOwnIO<S>, double, STAGE>(g, i, j, k, own, pml, h, dt, F[0][0], F[0][1], F[0][2], F[0][3],
+                                                   F[1][0], F[1][1], F[1][2], F[1][3], F[2][0], F[2][1], F[2][2],
+                                                   F[2][3], sidx >= 0, add);
 }
 
 // Copies the stored fields of the patch cells into an FP64 shadow set.
