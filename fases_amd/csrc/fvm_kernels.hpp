@@ -666,6 +666,10 @@ template <typename S, int STAGE>
 __device__ inline void boundary_cell_rk(int b, const Grid &g, const FieldSet<S> &in, const FieldSet<S> &out,
                                         const Bnd &bd, const AdeRk &ade, double Z0, double inv_2z0,
                                         const AdeRkBufs &ab, const Scl<double> &sc) {
+    // No contraction beyond the explicit fma calls: whether the compiler fuses a product
+    // into a following add depends on the basic blocks of the calling kernel, so the
+    // single-context and batched kernels would otherwise round differently.
+#pragma clang fp contract(off)
     typedef typename std::conditional<std::is_same<S, double>::value, double, float>::type R;
     typedef Conv<S, R> CV;
     const BndCell bc = bnd_cell(g, bd, b);
