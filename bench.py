@@ -327,21 +327,29 @@ def main() -> int:
         from fases_amd.solver import batch_run
         jobs = W.batch_jobs(args.batch_jobs)
         work = sum(int(np.prod(j.room)) * j.nsteps for j in jobs)
-        seq_ms, seq_ir = 0.0, []
+        # device time from events on each side, and host wall time around the same calls
+        # (run set-up, autotuning, graph capture and the IR copies included)
+        seq_ms, seq_ir, seq_wall = 0.0, [], 0.0
         for j in jobs:
             s1 = W.make_solver(j)
+            tw = time.time()
             r, _, el, _ = s1.run(j.nsteps)
+            seq_wall += time.time() - tw
             s1.close()
             seq_ms += el
             seq_ir.append(r)
         log("batched benchmark: %d jobs sequential %.0f ms" % (len(jobs), seq_ms))
         svs = [W.make_solver(j) for j in jobs]
+        tw = time.time()
         b_ir, b_ms = batch_run(svs, [j.nsteps for j in jobs])
+        b_wall = time.time() - tw
         for s1 in svs:
             s1.close()
         same = all(np.array_equal(a, b) for a, b in zip(b_ir, seq_ir))
         batch = {"jobs": len(jobs), "cell_steps": work, "batched_ms": round(b_ms, 1),
                  "sequential_ms": round(seq_ms, 1), "speedup": round(seq_ms / b_ms, 3),
+                 "batched_wall_s": round(b_wall, 2), "sequential_wall_s": round(seq_wall, 2),
+                 "speedup_wall": round(seq_wall / b_wall, 3),
                  "batched_value": round(work / (b_ms * 1e-3) / 1e6, 1),
                  "sequential_value": round(work / (seq_ms * 1e-3) / 1e6, 1), "unit": "Mcell-updates/s",
                  "irs_bitwise_equal": bool(same),

@@ -192,8 +192,9 @@ struct PatchAdjIn {
 
 // pv layout: [(ax * 10 + m) * nD + d], m < 5 the p-pair window slots, m >= 5 the velocity's
 // (slot-major, so that the lanes of a wave, consecutive d, read and write consecutive words).
+// Latency-bound gathers: held to 96 VGPRs for five waves per SIMD instead of four (98).
 template <typename S, typename R>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
 k_flux_adj_patch_part(Grid g, PatchAdjIn<S> in, const R *__restrict__ L0, const R *__restrict__ L1,
                       const R *__restrict__ L2, const R *__restrict__ L3, const double *__restrict__ lsh_out,
                       const uint8_t *__restrict__ upd, const int *__restrict__ dcells, int nD,
